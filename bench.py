@@ -1,0 +1,161 @@
+"""Benchmark: GA generations of the batched minimal-Tersoff Si workload.
+
+Workload (the batched Tersoff configuration): population 200, 1000 diamond-Si
+configurations of 8-512 atoms (log-uniform), +-3% axis strain, N(0, 0.1 A)
+displacements, 20% force configurations, labels from the ground-truth
+parameter set (computed by this library's find_force), cutoff 3.0 A.
+
+A step is one GA generation: the fitness of all 200 individuals over all
+configurations (one persistent, graph-replayed HIP launch) followed by the
+GA's sort / crossover / mutation.  The metric is individual x atom fitness
+evaluations per second.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload NAME]
+                  [--no-cpu-baseline] [--plain]
+
+Extra workloads (not the headline line): tersoff_small, rigid_ion, nn2b.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MI355X_FP32_PEAK = 157.3e12  # FLOP/s, vector FP32 (vendor figure)
+MI355X_HBM_PEAK = 8.0e12     # B/s
+# Algorithmic work per individual x atom evaluation (4 neighbours per atom):
+# pair terms ~106 FLOP x 4, triplets ~57 FLOP x 12, assembly -> ~1.1 kFLOP.
+TERSOFF_FLOP_PER_UNIT = 1.1e3
+RI_FLOP_PER_UNIT_PER_NEIGHBOR = 100.0
+
+WORKLOADS = {
+    "tersoff_batched": dict(kind="tersoff", Nc=1000, sizes="loguniform", P=200, seed=2),
+    "tersoff_small": dict(kind="tersoff", Nc=60, sizes="fixed", P=50, seed=1),
+    "rigid_ion": dict(kind="ri", Nc=500, P=200, seed=3),
+    "nn2b": dict(kind="nn2b", Nc=800, P=256, seed=5),
+}
+
+
+def make_workload(name):
+    import gpuga_amd
+    from gpuga_amd import synth
+    w = WORKLOADS[name]
+    if w["kind"] == "tersoff":
+        d = synth.tersoff_si(Nc=w["Nc"], sizes=w["sizes"], seed=w["seed"])
+    elif w["kind"] == "ri":
+        d = synth.fluorite_ab2(Nc=w["Nc"], seed=w["seed"])
+    else:
+        d = synth.wurtzite_ab(Nc=w["Nc"], seed=w["seed"])
+    unlabelled = gpuga_amd.Fitness(arrays=d)
+    d = synth.label(d, unlabelled.find_force)
+    unlabelled.close()
+    return d, w
+
+
+def cpu_baseline(d, w, genes, budget_s=15.0):
+    """FP64 oracle on the same training set, a bounded sample of individuals."""
+    from oracle import oracle as O
+    case = O.Case(potential=dict(potential_type=d["potential_type"], cutoff=d["cutoff"],
+                                 weights=d["weights"], pmin=d["pmin"], pmax=d["pmax"],
+                                 num_neurons=d.get("num_neurons", 0)),
+                  arrays=d)
+    N = case.N
+    t0 = time.perf_counter()
+    case.fitness(genes[:1], threads=1)
+    t1 = time.perf_counter() - t0
+    n = max(1, min(len(genes), int(budget_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    case.fitness(genes[:n], threads=1)
+    dt = time.perf_counter() - t0
+    return {"value": n * N / dt, "unit": "individual*atom evals/s", "cores": 1, "kind": "port",
+            "sample": f"{n} individuals x {N} atoms (full training set), FP64 scalar C oracle, "
+                      f"1 thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="tersoff_batched", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain", action="store_true", help="per-individual launches")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only; multi-GPU runs are not part of this build "
+              "(population sharding is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import gpuga_amd
+    from gpuga_amd import synth
+    d, w = make_workload(args.workload)
+    fit = gpuga_amd.Fitness(arrays=d, backend=1 if args.plain else 0)
+    P, V, N = w["P"], fit.V, fit.N
+    chunk = fit.autotune(P) if not args.plain else 0
+    ga = gpuga_amd.GA(P, P // 2, V, maximum_generation=args.warmup + args.steps,
+                      mutation_rate=0.2, seed=12345678)
+
+    def step(g):
+        genes = ga.population
+        f = fit.compute(genes, plain=args.plain)
+        ga.step(g, f)
+        return f
+
+    for g in range(args.warmup):
+        step(g)
+    fit.set_timing(True)
+    fit.timing(reset=True)
+    t0 = time.perf_counter()
+    best = None
+    for g in range(args.warmup, args.warmup + args.steps):
+        best = float(np.min(step(g)))
+    elapsed = time.perf_counter() - t0
+    kms, launches = fit.timing(reset=True)
+    ms_step = 1e3 * elapsed / args.steps
+    value = P * N * args.steps / elapsed
+    kernel_s = kms * 1e-3 / max(launches, 1)
+    if w["kind"] == "tersoff":
+        flop = TERSOFF_FLOP_PER_UNIT * P * N
+    elif w["kind"] == "ri":
+        flop = RI_FLOP_PER_UNIT_PER_NEIGHBOR * fit.bonds * P
+    else:
+        flop = 4096.0 * fit.bonds / 2 * P
+    if args.plain:
+        flop /= P  # timing covers one individual per launch group
+    achieved = flop / kernel_s
+    out = {
+        "metric": "individual*atom fitness evals/s",
+        "value": value,
+        "unit": "individual*atom evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 sums)",
+        "data": "synthetic (seeded diamond-Si cells, labels from ground-truth parameters)",
+        "config": {"workload": args.workload, "population": P, "configurations": fit.Nc,
+                   "atoms": N, "bonds": fit.bonds, "parameters": V,
+                   "backend": "per-individual" if args.plain else "population",
+                   "chunk_cost": chunk, "s_per_generation": ms_step / 1e3,
+                   "best_fitness": best},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": MI355X_FP32_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / MI355X_FP32_PEAK, "traffic": None,
+                     "kernel_ms": kernel_s * 1e3},
+    }
+    if not args.no_cpu_baseline and not args.plain:
+        genes = synth.random_genes(64, V, seed=99)
+        out["cpu_baseline"] = cpu_baseline(d, w, genes)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,448 @@
+// C ABI (include/gpuga.h): every entry point converts exceptions into a
+// status code and a thread-local message.
+#include "../../include/gpuga.h"
+
+#include <chrono>
+#include <cstring>
+#include <string>
+
+#include "fitness.h"
+#include "ga.h"
+#include "serve.h"
+
+using namespace gpuga;
+
+struct gpuga_fitness {
+  std::unique_ptr<Fitness> f;
+};
+struct gpuga_ga {
+  std::unique_ptr<GA> g;
+};
+struct gpuga_server {
+  std::unique_ptr<Server> s;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& fn)
+{
+  try {
+    fn();
+    return 0;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+  } catch (...) {
+    g_last_error = "unknown error";
+  }
+  return 1;
+}
+
+FitnessOptions to_options(const gpuga_options* o)
+{
+  FitnessOptions f;
+  if (!o) return f;
+  f.backend = o->backend;
+  f.chunk_cost = o->chunk_cost;
+  f.use_graph = o->use_graph;
+  f.nn2b_bf16 = o->nn2b_bf16;
+  if (o->nn2b_grid > 0) f.nn2b_grid = o->nn2b_grid;
+  f.verbose = o->verbose;
+  return f;
+}
+
+void need(const void* p, const char* what)
+{
+  if (!p) fail(std::string(what) + " is NULL");
+}
+
+} // namespace
+
+extern "C" {
+
+const char* gpuga_last_error(void) { return g_last_error.c_str(); }
+
+void gpuga_default_options(gpuga_options* opt)
+{
+  if (!opt) return;
+  FitnessOptions f;
+  opt->backend = f.backend;
+  opt->chunk_cost = f.chunk_cost;
+  opt->use_graph = f.use_graph;
+  opt->nn2b_bf16 = f.nn2b_bf16;
+  opt->nn2b_grid = f.nn2b_grid;
+  opt->verbose = f.verbose;
+}
+
+int gpuga_fitness_create(const char* input_dir, const gpuga_options* opt, gpuga_fitness** out)
+{
+  return guarded([&] {
+    need(input_dir, "input_dir");
+    need(out, "out");
+    auto h = new gpuga_fitness;
+    try {
+      h->f = std::make_unique<Fitness>(std::string(input_dir), to_options(opt));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int gpuga_fitness_create_arrays(int potential_type, int num_neurons, float cutoff,
+                                float weight_force, float weight_energy, float weight_virial,
+                                const float* pmin, const float* pmax, int Nc, int Nc_force,
+                                const int* Na, const int* type, const float* box9, const float* r,
+                                const float* pe_ref, const float* virial_ref,
+                                const float* force_ref, const gpuga_options* opt,
+                                gpuga_fitness** out)
+{
+  return guarded([&] {
+    need(out, "out");
+    need(Na, "Na");
+    need(type, "type");
+    need(box9, "box9");
+    need(r, "r");
+    PotentialSpec s;
+    s.type = potential_type;
+    s.num_neurons = num_neurons;
+    switch (potential_type) {
+    case POT_RI: s.V = 15; break;
+    case POT_TERSOFF1: s.V = 9; break;
+    case POT_TERSOFF2: s.V = 13; break;
+    case POT_NN2B:
+      if (num_neurons < 1) fail("num_neurons should >= 1.");
+      s.V = num_neurons * (num_neurons + 4) + 1;
+      break;
+    default: fail("unsupported potential type.");
+    }
+    s.cutoff = cutoff;
+    s.w_force = weight_force;
+    s.w_energy = weight_energy;
+    s.w_stress = weight_virial;
+    if (s.w_force < 0 || s.w_energy < 0 || s.w_stress < 0) fail("weights should >= 0");
+    s.pmin.resize(s.V);
+    s.pmax.resize(s.V);
+    s.names.resize(s.V);
+    for (int v = 0; v < s.V; ++v) {
+      if (potential_type == POT_NN2B) {
+        s.pmin[v] = -3.0f;
+        s.pmax[v] = 3.0f;
+      } else {
+        need(pmin, "pmin");
+        need(pmax, "pmax");
+        s.pmin[v] = pmin[v];
+        s.pmax[v] = pmax[v];
+      }
+    }
+    TrainSet ts = make_train(Nc, Nc_force, Na, type, box9, r, pe_ref, virial_ref, force_ref);
+    auto h = new gpuga_fitness;
+    try {
+      h->f = std::make_unique<Fitness>(s, std::move(ts), to_options(opt));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+void gpuga_fitness_destroy(gpuga_fitness* h) { delete h; }
+
+int gpuga_fitness_num_variables(const gpuga_fitness* h)
+{
+  return h && h->f ? h->f->number_of_variables() : -1;
+}
+
+int gpuga_fitness_bounds(const gpuga_fitness* h, float* pmin, float* pmax)
+{
+  return guarded([&] {
+    need(h, "handle");
+    const PotentialSpec& s = h->f->spec();
+    if (pmin) std::memcpy(pmin, s.pmin.data(), sizeof(float) * s.V);
+    if (pmax) std::memcpy(pmax, s.pmax.data(), sizeof(float) * s.V);
+  });
+}
+
+int gpuga_fitness_info(const gpuga_fitness* h, int* out)
+{
+  return guarded([&] {
+    need(h, "handle");
+    need(out, "out");
+    const TrainSet& t = h->f->trainset();
+    const PotentialSpec& s = h->f->spec();
+    out[0] = s.type;
+    out[1] = t.Nc;
+    out[2] = t.Nc_force;
+    out[3] = t.N;
+    out[4] = t.N_force;
+    out[5] = t.max_Na;
+    out[6] = h->f->neighbors().total;
+    out[7] = h->f->neighbors().max_nn;
+    out[8] = s.V;
+    out[9] = s.num_neurons;
+  });
+}
+
+int gpuga_fitness_compute(gpuga_fitness* h, int P, const float* genes, float* fitness)
+{
+  return gpuga_fitness_compute_terms(h, P, genes, fitness, nullptr);
+}
+
+int gpuga_fitness_compute_terms(gpuga_fitness* h, int P, const float* genes, float* fitness,
+                                double* terms)
+{
+  return guarded([&] {
+    need(h, "handle");
+    need(genes, "genes");
+    need(fitness, "fitness");
+    h->f->compute(P, genes, fitness, terms);
+  });
+}
+
+int gpuga_fitness_compute_plain(gpuga_fitness* h, int P, const float* genes, float* fitness,
+                                double* terms)
+{
+  return guarded([&] {
+    need(h, "handle");
+    need(genes, "genes");
+    need(fitness, "fitness");
+    h->f->compute_plain(P, genes, fitness, terms);
+  });
+}
+
+int gpuga_fitness_predict(gpuga_fitness* h, const char* dir, const float* elite)
+{
+  return guarded([&] {
+    need(h, "handle");
+    need(dir, "dir");
+    need(elite, "elite");
+    h->f->predict(dir, elite);
+  });
+}
+
+int gpuga_fitness_find_force(gpuga_fitness* h, const float* params, float* f, float* virial,
+                             float* pe)
+{
+  return guarded([&] {
+    need(h, "handle");
+    need(params, "params");
+    h->f->find_force(params, f, virial, pe);
+  });
+}
+
+int gpuga_fitness_neighbors(const gpuga_fitness* h, int* NN, int* start, int* NL)
+{
+  return guarded([&] {
+    need(h, "handle");
+    const NeighborList& nl = h->f->neighbors();
+    if (NN) std::memcpy(NN, nl.NN.data(), sizeof(int) * nl.NN.size());
+    if (start) std::memcpy(start, nl.start.data(), sizeof(int) * nl.start.size());
+    if (NL) std::memcpy(NL, nl.NL.data(), sizeof(int) * nl.NL.size());
+  });
+}
+
+int gpuga_inspect_neighbors(const char* dir, int* sizes, int* NN, int* start, int* NL)
+{
+  return guarded([&] {
+    need(dir, "input_dir");
+    need(sizes, "sizes");
+    const PotentialSpec s = read_potential(dir);
+    const TrainSet ts = read_train(dir);
+    const NeighborList nl = build_neighbors(ts, s.cutoff);
+    sizes[0] = ts.N;
+    sizes[1] = nl.total;
+    sizes[2] = nl.max_nn;
+    sizes[3] = ts.Nc;
+    if (NN) std::memcpy(NN, nl.NN.data(), sizeof(int) * nl.NN.size());
+    if (start) std::memcpy(start, nl.start.data(), sizeof(int) * nl.start.size());
+    if (NL) std::memcpy(NL, nl.NL.data(), sizeof(int) * nl.NL.size());
+  });
+}
+
+int gpuga_fitness_autotune(gpuga_fitness* h, int P, int* chunk_cost)
+{
+  return guarded([&] {
+    need(h, "handle");
+    const int c = h->f->autotune(P);
+    if (chunk_cost) *chunk_cost = c;
+  });
+}
+
+int gpuga_fitness_set_timing(gpuga_fitness* h, int on)
+{
+  return guarded([&] {
+    need(h, "handle");
+    h->f->set_timing(on != 0);
+  });
+}
+
+int gpuga_fitness_timing(gpuga_fitness* h, double* kernel_ms, long* launches, int reset)
+{
+  return guarded([&] {
+    need(h, "handle");
+    if (kernel_ms) *kernel_ms = h->f->kernel_ms();
+    if (launches) *launches = h->f->kernel_launches();
+    if (reset) h->f->reset_timers();
+  });
+}
+
+int gpuga_fitness_device_bytes(const gpuga_fitness* h, double* bytes)
+{
+  return guarded([&] {
+    need(h, "handle");
+    need(bytes, "bytes");
+    *bytes = (double)h->f->device_bytes();
+  });
+}
+
+// ------------------------------------------------------------------- GA
+
+int gpuga_ga_create(int population_size, int parent_number, int V, int maximum_generation,
+                    float mutation_rate, unsigned seed, gpuga_ga** out)
+{
+  return guarded([&] {
+    need(out, "out");
+    GAParams p;
+    p.population_size = population_size;
+    p.parent_number = parent_number;
+    p.maximum_generation = maximum_generation;
+    p.mutation_rate = mutation_rate;
+    p.seed = seed;
+    std::vector<float> lo(V, 0.0f), hi(V, 1.0f);
+    auto h = new gpuga_ga;
+    try {
+      h->g = std::make_unique<GA>(p, V, lo, hi);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+void gpuga_ga_destroy(gpuga_ga* g) { delete g; }
+
+int gpuga_ga_get_population(const gpuga_ga* g, float* genes)
+{
+  return guarded([&] {
+    need(g, "handle");
+    need(genes, "genes");
+    const auto& p = g->g->population();
+    std::memcpy(genes, p.data(), sizeof(float) * p.size());
+  });
+}
+
+int gpuga_ga_set_population(gpuga_ga* g, const float* genes)
+{
+  return guarded([&] {
+    need(g, "handle");
+    need(genes, "genes");
+    auto& p = g->g->population();
+    std::memcpy(p.data(), genes, sizeof(float) * p.size());
+  });
+}
+
+int gpuga_ga_step(gpuga_ga* g, int generation, const float* fitness, float* sorted)
+{
+  return guarded([&] {
+    need(g, "handle");
+    need(fitness, "fitness");
+    auto& f = g->g->fitness();
+    std::memcpy(f.data(), fitness, sizeof(float) * f.size());
+    g->g->sort_population();
+    if (sorted) std::memcpy(sorted, f.data(), sizeof(float) * f.size());
+    g->g->crossover();
+    g->g->mutation(generation);
+  });
+}
+
+int gpuga_run(const char* input_dir, const gpuga_options* opt, double* seconds_per_generation)
+{
+  return guarded([&] {
+    need(input_dir, "input_dir");
+    const std::string dir(input_dir);
+    Fitness fit(dir, to_options(opt));
+    GA ga(read_ga(dir), fit.number_of_variables(), fit.spec().pmin, fit.spec().pmax);
+    const auto t0 = std::chrono::steady_clock::now();
+    ga.run(dir, [&](int P, const float* genes, float* fitness) {
+      fit.compute(P, genes, fitness);
+    });
+    const auto t1 = std::chrono::steady_clock::now();
+    if (seconds_per_generation)
+      *seconds_per_generation = std::chrono::duration<double>(t1 - t0).count() /
+                                ga.params().maximum_generation;
+    fit.predict(dir, ga.population().data());
+  });
+}
+
+// -------------------------------------------------------------- serving
+
+int gpuga_server_create(double budget, int max_active, const gpuga_options* opt,
+                        gpuga_server** out)
+{
+  return guarded([&] {
+    need(out, "out");
+    auto h = new gpuga_server;
+    h->s = std::make_unique<Server>(budget, max_active, to_options(opt));
+    *out = h;
+  });
+}
+
+void gpuga_server_destroy(gpuga_server* s) { delete s; }
+
+int gpuga_server_submit(gpuga_server* s, const char* dir, int write_outputs, int* job_id)
+{
+  return guarded([&] {
+    need(s, "handle");
+    need(dir, "input_dir");
+    const int id = s->s->submit(dir, write_outputs != 0);
+    if (job_id) *job_id = id;
+  });
+}
+
+int gpuga_server_drain(gpuga_server* s)
+{
+  return guarded([&] {
+    need(s, "handle");
+    s->s->drain();
+  });
+}
+
+int gpuga_server_poll(gpuga_server* s, int* active, int* queued, int* finished)
+{
+  return guarded([&] {
+    need(s, "handle");
+    s->s->poll();
+    if (active) *active = s->s->active();
+    if (queued) *queued = s->s->queued();
+    if (finished) *finished = s->s->finished();
+  });
+}
+
+int gpuga_server_job(const gpuga_server* s, int id, int* state, int* generations,
+                     float* best_fitness, float* elite)
+{
+  return guarded([&] {
+    need(s, "handle");
+    s->s->job_info(id, state, generations, best_fitness, elite);
+  });
+}
+
+int gpuga_server_latencies(const gpuga_server* s, double* ms, int* n)
+{
+  return guarded([&] {
+    need(s, "handle");
+    need(n, "n");
+    const std::vector<double>& v = s->s->latencies();
+    const int m = std::min<int>(*n, (int)v.size());
+    if (ms)
+      for (int i = 0; i < m; ++i) ms[i] = v[i];
+    *n = (int)v.size();
+  });
+}
+
+} // extern "C"

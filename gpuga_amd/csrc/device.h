@@ -1,0 +1,53 @@
+// Device-side views of a resident training set and the shared kernel helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "params.h"
+
+namespace gpuga {
+
+constexpr int WAVE = 64; // CDNA wavefront width
+
+// Read-only training set in HBM: per-atom SoA arrays, CSR neighbour list and
+// the parameter-independent geometry tables.
+struct DevSet {
+  int Nc, Nc_force, N, N_force;
+  const int* Na;          // [Nc]
+  const int* Na_sum;      // [Nc]
+  const int* type;        // [N]
+  const int* nn;          // [N] neighbour count
+  const int* bond0;       // [N] first bond of atom i
+  const int* nl;          // [B] neighbour atom of bond b
+  const int* rev;         // [B] reverse bond of bond b
+  const int* ptype;       // [B] type_i + type_j of bond b
+  const float4* bond;     // [B] r_ij.x, r_ij.y, r_ij.z, |r_ij|
+  const int* tri0;        // [N] first triplet of atom i (Tersoff only)
+  const float4* tri;      // [T] cos theta, d cos / d r_ij (Tersoff only)
+  const float* pe_ref;    // [Nc]
+  const float* virial_ref;// [6][Nc]
+  const float* force_ref; // [3][N]
+};
+
+// Normalisers and weights of the fitness (src/fitness.cu:329-331).
+struct FitnessNorm {
+  double inv_pss, inv_vss, inv_fss;
+  double w_energy, w_stress, w_force;
+};
+
+// Genes in [0,1] to physical parameters, float arithmetic with no fused
+// multiply-add, exactly as src/fitness.cu:319-324.
+__device__ inline float denormalise(float lo, float hi, float x)
+{
+#pragma clang fp contract(off)
+  float b = hi - lo;
+  float bx = b * x;
+  return lo + bx;
+}
+
+// Broadcast lane 0's value to the whole wavefront.
+__device__ inline int wave_first(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+} // namespace gpuga

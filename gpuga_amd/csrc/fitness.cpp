@@ -1,0 +1,706 @@
+// Fitness evaluator host code: set-up, plug-ins, population plans, predict.
+#include "fitness.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace gpuga {
+
+int tersoff_population_occupancy(int block);
+int ri_population_occupancy(int block);
+int nn2b_population_occupancy(int block);
+
+// ------------------------------------------------------------ plug-ins
+
+namespace {
+
+// Host denormalisation with float arithmetic and no fused multiply-add,
+// exactly as src/fitness.cu:319-324 (this file is compiled with
+// -ffp-contract=off).
+void denormalise_host(const PotentialSpec& s, const float* genes, float* out)
+{
+  for (int m = 0; m < s.V; ++m) {
+    const float a = s.pmin[m];
+    const float b = s.pmax[m] - a;
+    const float bx = b * genes[m];
+    out[m] = a + bx;
+  }
+}
+
+class TersoffPotential : public Potential {
+public:
+  explicit TersoffPotential(int num_types) : num_types_(num_types) {}
+  void initialize(int, int) override {}
+  void init_bonds(int B)
+  {
+    b_.resize(std::max(B, 1));
+    bp_.resize(std::max(B, 1));
+    f12_.resize(3 * (size_t)std::max(B, 1));
+  }
+  void update_potential(const std::vector<float>& p) override
+  {
+    q_ = tersoff_params(num_types_, p.data());
+  }
+  void find_force(const DevSet& s, float* f, float* virial, float* pe, hipStream_t st) override
+  {
+    launch_tersoff_plain(s, q_, b_.get(), bp_.get(), f12_.get(), f, virial, pe, st);
+  }
+
+private:
+  int num_types_;
+  TersoffParams q_{};
+  DeviceBuffer<float> b_, bp_, f12_;
+};
+
+class RIPotential : public Potential {
+public:
+  explicit RIPotential(const float4* tab) : tab_(tab) {}
+  void initialize(int, int) override {}
+  void update_potential(const std::vector<float>& p) override { q_ = ri_params(p.data()); }
+  void find_force(const DevSet& s, float* f, float* virial, float* pe, hipStream_t st) override
+  {
+    launch_ri_plain(s, q_, tab_, f, virial, pe, st);
+  }
+
+private:
+  const float4* tab_;
+  RIParams q_{};
+};
+
+class NN2BPotential : public Potential {
+public:
+  explicit NN2BPotential(int n) : n_(n) {}
+  void initialize(int, int) override { w_.resize(n_ * (n_ + 4) + 1); }
+  void update_potential(const std::vector<float>& p) override
+  {
+    host_ = p;
+    dirty_ = true;
+  }
+  void find_force(const DevSet& s, float* f, float* virial, float* pe, hipStream_t st) override
+  {
+    if (dirty_) {
+      w_.upload(host_, st);
+      dirty_ = false;
+    }
+    launch_nn2b_plain(s, n_, w_.get(), f, virial, pe, st);
+  }
+
+private:
+  int n_;
+  std::vector<float> host_;
+  bool dirty_ = false;
+  DeviceBuffer<float> w_;
+};
+
+} // namespace
+
+std::unique_ptr<Potential> make_potential(const PotentialSpec& spec, const DeviceSet& ds)
+{
+  switch (spec.type) {
+  case POT_TERSOFF1:
+  case POT_TERSOFF2: {
+    auto p = std::make_unique<TersoffPotential>(spec.type == POT_TERSOFF2 ? 2 : 1);
+    p->init_bonds((int)ds.bond.size());
+    return p;
+  }
+  case POT_RI: return std::make_unique<RIPotential>(ds.ri_tab.get());
+  case POT_NN2B: {
+    auto p = std::make_unique<NN2BPotential>(spec.num_neurons);
+    p->initialize(0, 0);
+    return p;
+  }
+  }
+  fail("unsupported potential type.");
+}
+
+size_t DeviceSet::bytes() const
+{
+  return Na.bytes() + Na_sum.bytes() + type.bytes() + nn.bytes() + bond0.bytes() + nl.bytes() +
+         rev.bytes() + ptype.bytes() + tri0.bytes() + bond.bytes() + tri.bytes() +
+         ri_tab.bytes() + pe_ref.bytes() + virial_ref.bytes() + force_ref.bytes();
+}
+
+PopPlan::~PopPlan()
+{
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+}
+
+// ------------------------------------------------------------ set-up
+
+Fitness::Fitness(const std::string& dir, const FitnessOptions& opt) : opt_(opt)
+{
+  spec_ = read_potential(dir);
+  ts_ = read_train(dir);
+  setup();
+}
+
+Fitness::Fitness(const PotentialSpec& spec, TrainSet ts, const FitnessOptions& opt)
+  : opt_(opt), spec_(spec), ts_(std::move(ts))
+{
+  setup();
+}
+
+Fitness::~Fitness()
+{
+  plans_.clear();
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Fitness::setup()
+{
+  int ndev = 0;
+  GPUGA_HIP(hipGetDeviceCount(&ndev));
+  if (ndev < 1) fail("no HIP device available");
+  GPUGA_HIP(hipGetDevice(&device_));
+  hipDeviceProp_t prop;
+  GPUGA_HIP(hipGetDeviceProperties(&prop, device_));
+  num_cu_ = prop.multiProcessorCount;
+  GPUGA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  GPUGA_HIP(hipEventCreate(&ev0_));
+  GPUGA_HIP(hipEventCreate(&ev1_));
+
+  const bool tersoff = spec_.type == POT_TERSOFF1 || spec_.type == POT_TERSOFF2;
+  if (spec_.type == POT_RI)
+    for (int t : ts_.type)
+      if (t != 0 && t != 1) fail("RI potential needs atom types 0 and 1");
+  nl_ = build_neighbors(ts_, spec_.cutoff);
+  for (int i = 0; i < ts_.N; ++i)
+    for (int s = 0; s < nl_.NN[i]; ++s)
+      if (nl_.NL[nl_.rev[nl_.start[i] + s]] != i)
+        fail("neighbour list is not symmetric (reverse bond missing)");
+  geo_ = build_geometry(ts_, nl_, tersoff);
+  dmin_ = 1e30f;
+  dmax_ = 0.0f;
+  for (const Vec4& b : geo_.bond) {
+    dmin_ = std::min(dmin_, b.w);
+    dmax_ = std::max(dmax_, b.w);
+  }
+  if (geo_.bond.empty()) dmin_ = dmax_ = 1.0f;
+
+  std::vector<int> ptype(nl_.total);
+  for (int i = 0; i < ts_.N; ++i)
+    for (int s = 0; s < nl_.NN[i]; ++s) {
+      const int b = nl_.start[i] + s, j = nl_.NL[b];
+      ptype[b] = tersoff ? (ts_.type[i] != ts_.type[j] ? 1 : 0) : ts_.type[i] + ts_.type[j];
+    }
+
+  hipStream_t st = stream_;
+  ds_.Na.upload(ts_.Na, st);
+  ds_.Na_sum.upload(ts_.Na_sum, st);
+  ds_.type.upload(ts_.type, st);
+  ds_.nn.upload(nl_.NN, st);
+  ds_.bond0.upload(nl_.start, st);
+  ds_.nl.upload(nl_.NL.empty() ? std::vector<int>{0} : nl_.NL, st);
+  ds_.rev.upload(nl_.rev.empty() ? std::vector<int>{0} : nl_.rev, st);
+  ds_.ptype.upload(ptype.empty() ? std::vector<int>{0} : ptype, st);
+  static_assert(sizeof(Vec4) == sizeof(float4), "layout");
+  if (geo_.bond.empty()) geo_.bond.push_back(Vec4{0, 0, 0, 1});
+  ds_.bond.upload((const float4*)geo_.bond.data(), geo_.bond.size(), st);
+  if (tersoff) {
+    ds_.tri0.upload(geo_.tri_start, st);
+    if (geo_.tri.empty()) geo_.tri.push_back(Vec4{0, 0, 0, 0});
+    ds_.tri.upload((const float4*)geo_.tri.data(), geo_.tri.size(), st);
+  } else {
+    ds_.tri0.upload(std::vector<int>(ts_.N, 0), st);
+    ds_.tri.upload(std::vector<float4>(1), st);
+  }
+  if (spec_.type == POT_RI) {
+    // per-bond factors of src/ri.cu:80-101 that do not depend on parameters
+    std::vector<float4> tab(std::max(nl_.total, 1));
+    for (int b = 0; b < nl_.total; ++b) {
+      const double d = geo_.bond[b].w;
+      const double dinv = 1.0 / d;
+      const double e = std::erfc(0.2 * d) * dinv;
+      const double g = e * dinv + 0.225675833419103 * std::exp(-0.04 * d * d) * dinv;
+      tab[b] = make_float4((float)std::pow(dinv, 6), (float)e, (float)g, (float)dinv);
+    }
+    ds_.ri_tab.upload(tab, st);
+  }
+  ds_.pe_ref.upload(ts_.pe_ref, st);
+  ds_.virial_ref.upload(ts_.virial_ref, st);
+  ds_.force_ref.upload(ts_.force_ref, st);
+  GPUGA_HIP(hipStreamSynchronize(st));
+
+  DevSet& v = ds_.view;
+  v.Nc = ts_.Nc;
+  v.Nc_force = ts_.Nc_force;
+  v.N = ts_.N;
+  v.N_force = ts_.N_force;
+  v.Na = ds_.Na.get();
+  v.Na_sum = ds_.Na_sum.get();
+  v.type = ds_.type.get();
+  v.nn = ds_.nn.get();
+  v.bond0 = ds_.bond0.get();
+  v.nl = ds_.nl.get();
+  v.rev = ds_.rev.get();
+  v.ptype = ds_.ptype.get();
+  v.bond = ds_.bond.get();
+  v.tri0 = ds_.tri0.get();
+  v.tri = ds_.tri.get();
+  v.pe_ref = ds_.pe_ref.get();
+  v.virial_ref = ds_.virial_ref.get();
+  v.force_ref = ds_.force_ref.get();
+
+  potential_ = make_potential(spec_, ds_);
+  potential_->initialize(ts_.N, ts_.max_Na);
+  d_pmin_.upload(spec_.pmin, st);
+  d_pmax_.upload(spec_.pmax, st);
+  GPUGA_HIP(hipStreamSynchronize(st));
+  d_f_.resize(3 * (size_t)ts_.N);
+  d_virial_.resize(6 * (size_t)ts_.N);
+  d_pe_.resize(ts_.N);
+  d_cerr_.resize(3 * (size_t)ts_.Nc);
+  d_fit1_.resize(1);
+  d_terms1_.resize(3);
+}
+
+NN2BGrid Fitness::nn2b_grid_desc() const
+{
+  NN2BGrid g;
+  g.G = std::max(opt_.nn2b_grid, 16);
+  g.n = spec_.num_neurons;
+  g.bf16 = opt_.nn2b_bf16;
+  // one spare cell on each side of the training set's pair-distance range
+  const float span = std::max(dmax_ - dmin_, 1e-3f);
+  g.h = span / (float)(g.G - 3);
+  g.d0 = dmin_ - g.h;
+  return g;
+}
+
+FitnessNorm Fitness::norm() const
+{
+  FitnessNorm n;
+  n.inv_pss = 1.0 / ts_.potential_square_sum;
+  n.inv_vss = 1.0 / ts_.virial_square_sum;
+  n.inv_fss = 1.0 / ts_.force_square_sum;
+  n.w_energy = spec_.w_energy;
+  n.w_stress = spec_.w_stress;
+  n.w_force = spec_.w_force;
+  return n;
+}
+
+size_t Fitness::device_bytes() const
+{
+  size_t b = ds_.bytes() + d_f_.bytes() + d_virial_.bytes() + d_pe_.bytes() + d_cerr_.bytes();
+  for (auto& kv : plans_) {
+    const PopPlan& p = *kv.second;
+    b += p.items.bytes() + p.part.bytes() + p.err.bytes() + p.bondf.bytes() + p.genes.bytes() +
+         p.fitness.bytes() + p.terms.bytes() + p.counters.bytes() + p.nn2b_table.bytes();
+  }
+  return b;
+}
+
+// ------------------------------------------------------- population plans
+
+namespace {
+
+long long atom_cost(int type, int nn, bool force_cfg)
+{
+  if (type == POT_TERSOFF1 || type == POT_TERSOFF2) return (long long)nn * nn + 4LL * nn + 8;
+  return (force_cfg ? nn : nn / 2 + 1) + 4;
+}
+
+int nn_class(int max_nn)
+{
+  if (max_nn <= 4) return 0;
+  if (max_nn <= 8) return 1;
+  return 2;
+}
+
+} // namespace
+
+size_t Fitness::plan_bytes(int P) const
+{
+  const size_t K = (P + WAVE - 1) / WAVE;
+  const size_t B_force = ts_.N_force > 0 ? (size_t)nl_.start[ts_.N_force - 1] +
+                                             nl_.NN[ts_.N_force - 1]
+                                         : 0;
+  const bool tersoff = spec_.type == POT_TERSOFF1 || spec_.type == POT_TERSOFF2;
+  size_t b = K * 3 * ts_.Nc * WAVE * sizeof(double);              // err
+  b += (size_t)2 * ts_.Nc * K * 8 * WAVE * sizeof(double);         // part (typical)
+  if (tersoff) b += K * B_force * 3 * WAVE * sizeof(float);       // bondf
+  if (spec_.type == POT_NN2B) b += (size_t)3 * opt_.nn2b_grid * K * WAVE * sizeof(float);
+  return b;
+}
+
+void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
+{
+  const int Nc = ts_.Nc;
+  const bool tersoff = spec_.type == POT_TERSOFF1 || spec_.type == POT_TERSOFF2;
+  if (tersoff && nl_.max_nn > 16)
+    fail(strfmt("population kernel supports at most 16 neighbours per atom (found %d); "
+                "use the per-individual backend",
+                nl_.max_nn));
+  pl.P = P;
+  pl.K = (P + WAVE - 1) / WAVE;
+  pl.chunk_cost = chunk_cost;
+  const int K = pl.K;
+
+  struct Chunk {
+    int c, chunk, a0, a1, cls;
+    long long cost;
+  };
+  std::vector<Chunk> chunks;
+  std::vector<int> nch(Nc), slot0(Nc);
+  int slots = 0;
+  for (int c = 0; c < Nc; ++c) {
+    const int a0 = ts_.Na_sum[c], a1 = a0 + ts_.Na[c];
+    const bool fc = c < ts_.Nc_force;
+    long long total = 0;
+    for (int i = a0; i < a1; ++i) total += atom_cost(spec_.type, nl_.NN[i], fc);
+    int n = (int)std::max<long long>(1, (total + chunk_cost - 1) / std::max(chunk_cost, 1));
+    n = std::min(n, a1 - a0);
+    nch[c] = n;
+    slot0[c] = slots;
+    slots += n;
+    long long acc = 0;
+    int start = a0, k = 0, mx = 0;
+    long long cacc = 0;
+    for (int i = a0; i < a1; ++i) {
+      const long long ci = atom_cost(spec_.type, nl_.NN[i], fc);
+      acc += ci;
+      cacc += ci;
+      mx = std::max(mx, nl_.NN[i]);
+      const bool cut = (k < n - 1) && (acc * n >= total * (k + 1)) && (a1 - (i + 1) >= n - 1 - k);
+      if (cut || i == a1 - 1) {
+        chunks.push_back(Chunk{c, k, start, i + 1, nn_class(mx), cacc});
+        start = i + 1;
+        ++k;
+        mx = 0;
+        cacc = 0;
+      }
+    }
+    if (k != n) fail("internal: chunk split mismatch");
+  }
+  std::stable_sort(chunks.begin(), chunks.end(),
+                   [](const Chunk& a, const Chunk& b) { return a.cost > b.cost; });
+  std::vector<WorkItem> items;
+  items.reserve(chunks.size() * K);
+  for (const Chunk& ch : chunks)
+    for (int k = 0; k < K; ++k)
+      items.push_back(WorkItem{ch.c, k, ch.chunk, ch.a0, ch.a1, ch.cls, 0, 0});
+  pl.n_items = (int)items.size();
+
+  hipStream_t st = stream_;
+  pl.items.upload(items, st);
+  pl.cfg_nchunks.upload(nch, st);
+  pl.cfg_slot0.upload(slot0, st);
+  pl.part.resize((size_t)std::max(slots, 1) * K * 8 * WAVE);
+  pl.err.resize((size_t)K * 3 * Nc * WAVE);
+  pl.terms.resize(3 * (size_t)P);
+  pl.fitness.resize(P);
+  pl.genes.resize((size_t)P * spec_.V);
+  const int B_force =
+    ts_.N_force > 0 ? nl_.start[ts_.N_force - 1] + nl_.NN[ts_.N_force - 1] : 0;
+  pl.bondf.resize(tersoff ? std::max<size_t>((size_t)K * B_force * 3 * WAVE, 1) : 1);
+  if (spec_.type == POT_NN2B) pl.nn2b_table.resize((size_t)K * nn2b_grid_desc().G * 3 * WAVE);
+  pl.counters.resize((size_t)Nc * K + K + 1);
+  pl.counters.zero(st);
+  pl.h_genes.resize((size_t)P * spec_.V);
+  pl.h_fitness.resize(P);
+  pl.h_terms.resize(3 * (size_t)P);
+
+  PopCtl& c = pl.ctl;
+  c.items = pl.items.get();
+  c.n_items = pl.n_items;
+  c.K = K;
+  c.P = P;
+  c.V = spec_.V;
+  c.cfg_nchunks = pl.cfg_nchunks.get();
+  c.cfg_slot0 = pl.cfg_slot0.get();
+  c.genes = pl.genes.get();
+  c.pmin = d_pmin_.get();
+  c.pmax = d_pmax_.get();
+  c.part = pl.part.get();
+  c.bondf = pl.bondf.get();
+  c.B_force = B_force;
+  c.err = pl.err.get();
+  c.cnt = pl.counters.get();
+  c.blk_done = pl.counters.get() + (size_t)Nc * K;
+  c.fetch = pl.counters.get() + (size_t)Nc * K + K;
+  c.fitness = pl.fitness.get();
+  c.terms = pl.terms.get();
+  c.norm = norm();
+
+  int occ = opt_.blocks_per_cu;
+  if (occ <= 0) {
+    switch (spec_.type) {
+    case POT_RI: occ = ri_population_occupancy(pl.block); break;
+    case POT_NN2B: occ = nn2b_population_occupancy(pl.block); break;
+    default: occ = tersoff_population_occupancy(pl.block); break;
+    }
+    occ = std::max(occ, 1);
+  }
+  const int waves_per_block = pl.block / WAVE;
+  const int need = (pl.n_items + waves_per_block - 1) / waves_per_block;
+  pl.grid = std::max(1, std::min(need, num_cu_ * occ));
+  GPUGA_HIP(hipStreamSynchronize(st));
+}
+
+} // namespace gpuga
+
+namespace gpuga {
+
+PopPlan& Fitness::plan(int P)
+{
+  auto it = plans_.find(P);
+  if (it != plans_.end()) return *it->second;
+  int cost = opt_.chunk_cost;
+  if (cost <= 0) {
+    auto t = tuned_.find(P);
+    cost = t != tuned_.end() ? t->second : autotune(P);
+  }
+  auto pl = std::make_unique<PopPlan>();
+  build_plan(*pl, P, cost);
+  PopPlan& ref = *pl;
+  plans_[P] = std::move(pl);
+  return ref;
+}
+
+void Fitness::launch_population(PopPlan& pl, hipStream_t st)
+{
+  switch (spec_.type) {
+  case POT_RI: launch_ri_population(pl.ctl, ds_.view, ds_.ri_tab.get(), pl.grid, pl.block, st); break;
+  case POT_NN2B: {
+    NN2BGrid g = nn2b_grid_desc();
+    launch_nn2b_tabulate(pl.ctl, g, pl.nn2b_table.get(), st);
+    launch_nn2b_population(pl.ctl, ds_.view, g, pl.nn2b_table.get(), pl.grid, pl.block, st);
+    break;
+  }
+  default: launch_tersoff_population(pl.ctl, ds_.view, pl.grid, pl.block, st); break;
+  }
+}
+
+void Fitness::compute_async(int P, const float* genes)
+{
+  if (P < 1) fail("population size must be positive");
+  PopPlan& pl = plan(P);
+  hipStream_t st = stream_;
+  std::memcpy(pl.h_genes.get(), genes, sizeof(float) * (size_t)P * spec_.V);
+  GPUGA_HIP(hipMemcpyAsync(pl.genes.get(), pl.h_genes.get(), sizeof(float) * (size_t)P * spec_.V,
+                           hipMemcpyHostToDevice, st));
+  if (timing_) GPUGA_HIP(hipEventRecord(ev0_, st));
+  if (opt_.use_graph) {
+    if (!pl.exec) {
+      GPUGA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      launch_population(pl, st);
+      GPUGA_HIP(hipStreamEndCapture(st, &pl.graph));
+      GPUGA_HIP(hipGraphInstantiate(&pl.exec, pl.graph, nullptr, nullptr, 0));
+    }
+    GPUGA_HIP(hipGraphLaunch(pl.exec, st));
+  } else {
+    launch_population(pl, st);
+  }
+  if (timing_) GPUGA_HIP(hipEventRecord(ev1_, st));
+  GPUGA_HIP(hipMemcpyAsync(pl.h_fitness.get(), pl.fitness.get(), sizeof(float) * P,
+                           hipMemcpyDeviceToHost, st));
+  GPUGA_HIP(hipMemcpyAsync(pl.h_terms.get(), pl.terms.get(), sizeof(double) * 3 * P,
+                           hipMemcpyDeviceToHost, st));
+  pending_P_ = P;
+}
+
+void Fitness::compute_wait(float* fitness, double* terms)
+{
+  const int P = pending_P_;
+  if (P < 1) fail("no generation in flight");
+  GPUGA_HIP(hipStreamSynchronize(stream_));
+  PopPlan& pl = *plans_.at(P);
+  if (timing_) {
+    float ms = 0.0f;
+    GPUGA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    kernel_ms_ += ms;
+    ++kernel_launches_;
+  }
+  if (fitness) std::memcpy(fitness, pl.h_fitness.get(), sizeof(float) * P);
+  if (terms) std::memcpy(terms, pl.h_terms.get(), sizeof(double) * 3 * P);
+  pending_P_ = 0;
+}
+
+void Fitness::compute(int P, const float* genes, float* fitness, double* terms)
+{
+  if (opt_.backend == 1) {
+    compute_plain(P, genes, fitness, terms);
+    return;
+  }
+  compute_async(P, genes);
+  compute_wait(fitness, terms);
+}
+
+void Fitness::compute_plain(int P, const float* genes, float* fitness, double* terms)
+{
+  // src/fitness.cu:316-333: one parameter set at a time
+  std::vector<float> params(spec_.V);
+  const FitnessNorm nrm = norm();
+  hipStream_t st = stream_;
+  for (int n = 0; n < P; ++n) {
+    denormalise_host(spec_, genes + (size_t)n * spec_.V, params.data());
+    potential_->update_potential(params);
+    if (timing_) GPUGA_HIP(hipEventRecord(ev0_, st));
+    potential_->find_force(ds_.view, d_f_.get(), d_virial_.get(), d_pe_.get(), st);
+    launch_fitness_terms(ds_.view, nrm, d_f_.get(), d_virial_.get(), d_pe_.get(), d_cerr_.get(),
+                         d_fit1_.get(), d_terms1_.get(), st);
+    if (timing_) GPUGA_HIP(hipEventRecord(ev1_, st));
+    float fit = 0.0f;
+    double t[3];
+    GPUGA_HIP(hipMemcpyAsync(&fit, d_fit1_.get(), sizeof(float), hipMemcpyDeviceToHost, st));
+    GPUGA_HIP(hipMemcpyAsync(t, d_terms1_.get(), sizeof(t), hipMemcpyDeviceToHost, st));
+    GPUGA_HIP(hipStreamSynchronize(st));
+    if (timing_) {
+      float ms = 0.0f;
+      GPUGA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+      kernel_ms_ += ms;
+      ++kernel_launches_;
+    }
+    fitness[n] = fit;
+    if (terms) std::memcpy(terms + 3 * (size_t)n, t, sizeof(t));
+  }
+}
+
+void Fitness::find_force(const float* physical, float* f, float* virial, float* pe)
+{
+  std::vector<float> params(physical, physical + spec_.V);
+  potential_->update_potential(params);
+  hipStream_t st = stream_;
+  potential_->find_force(ds_.view, d_f_.get(), d_virial_.get(), d_pe_.get(), st);
+  GPUGA_HIP(hipGetLastError());
+  if (f) d_f_.download(f, 3 * (size_t)ts_.N, st);
+  if (virial) d_virial_.download(virial, 6 * (size_t)ts_.N, st);
+  if (pe) d_pe_.download(pe, ts_.N, st);
+  GPUGA_HIP(hipStreamSynchronize(st));
+}
+
+void Fitness::predict(const std::string& dir, const float* elite)
+{
+  // src/fitness.cu:335-390
+  std::vector<float> params(spec_.V);
+  denormalise_host(spec_, elite, params.data());
+  const size_t N = ts_.N;
+  std::vector<float> f(3 * N), w(6 * N), e(N);
+  find_force(params.data(), f.data(), w.data(), e.data());
+  auto open = [&](const char* name) {
+    std::string path = dir + "/" + name;
+    FILE* fid = std::fopen(path.c_str(), "w");
+    if (!fid) fail(strfmt("Failed to open %s!", path.c_str()));
+    return fid;
+  };
+  FILE* ff = open("force.out");
+  for (int n = 0; n < ts_.N_force; ++n)
+    std::fprintf(ff, "%g %g %g %g %g %g\n", f[n], f[n + N], f[n + N * 2], ts_.force_ref[n],
+                 ts_.force_ref[n + N], ts_.force_ref[n + N * 2]);
+  std::fclose(ff);
+  auto per_config = [&](FILE* fid, const float* data, const float* ref) {
+    for (int c = ts_.Nc_force; c < ts_.Nc; ++c) {
+      double sum = 0.0;
+      for (int m = 0; m < ts_.Na[c]; ++m) sum += data[ts_.Na_sum[c] + m];
+      std::fprintf(fid, "%g %g\n", (float)sum, ref[c]);
+    }
+  };
+  FILE* fe = open("energy.out");
+  per_config(fe, e.data(), ts_.pe_ref.data());
+  std::fclose(fe);
+  FILE* fv = open("virial.out");
+  for (int k = 0; k < 6; ++k) per_config(fv, w.data() + N * k, ts_.virial_ref.data() + ts_.Nc * k);
+  std::fclose(fv);
+}
+
+int Fitness::heuristic_chunk_cost(int P) const
+{
+  long long total = 0, maxcfg = 0;
+  for (int c = 0; c < ts_.Nc; ++c) {
+    long long t = 0;
+    for (int i = ts_.Na_sum[c]; i < ts_.Na_sum[c] + ts_.Na[c]; ++i)
+      t += atom_cost(spec_.type, nl_.NN[i], c < ts_.Nc_force);
+    total += t;
+    maxcfg = std::max(maxcfg, t);
+  }
+  const long long K = (P + WAVE - 1) / WAVE;
+  long long c = total * K / std::max<long long>(1, 16LL * num_cu_);
+  c = std::max<long long>(c, 64);
+  c = std::min<long long>(c, std::max<long long>(maxcfg, 64));
+  return (int)c;
+}
+
+size_t Fitness::estimate_bytes(const PotentialSpec& s, const TrainSet& ts, int P,
+                               const FitnessOptions& opt)
+{
+  const NeighborList nl = build_neighbors(ts, s.cutoff);
+  const size_t N = ts.N, Nc = ts.Nc, B = std::max(nl.total, 1);
+  const bool tersoff = s.type == POT_TERSOFF1 || s.type == POT_TERSOFF2;
+  size_t tri = 0;
+  for (int i = 0; i < ts.N; ++i) tri += (size_t)nl.NN[i] * nl.NN[i];
+  size_t b = N * sizeof(int) * 5 + Nc * sizeof(int) * 2 + B * (3 * sizeof(int) + 16);
+  b += (tersoff ? tri * 16 + B * 4 * 5 : 0) + (s.type == POT_RI ? B * 16 : 0);
+  b += Nc * 7 * sizeof(float) + N * 3 * sizeof(float);  // references
+  b += N * 10 * sizeof(float) + Nc * 3 * sizeof(double); // per-atom outputs
+  const size_t K = (P + WAVE - 1) / WAVE;
+  const size_t B_force = ts.N_force > 0 ? (size_t)nl.start[ts.N_force - 1] + nl.NN[ts.N_force - 1] : 0;
+  b += K * 3 * Nc * WAVE * sizeof(double) + 2 * Nc * K * 8 * WAVE * sizeof(double);
+  if (tersoff) b += K * B_force * 3 * WAVE * sizeof(float);
+  if (s.type == POT_NN2B) b += (size_t)3 * opt.nn2b_grid * K * WAVE * sizeof(float);
+  b += (size_t)P * s.V * sizeof(float) + (size_t)P * 4 * sizeof(double);
+  return b;
+}
+
+double Fitness::time_plan(int P, int chunk_cost, int reps)
+{
+  PopPlan pl;
+  build_plan(pl, P, chunk_cost);
+  std::vector<float> genes((size_t)P * spec_.V);
+  for (size_t i = 0; i < genes.size(); ++i) genes[i] = (float)((i * 2654435761u) % 1000) / 1000.0f;
+  hipStream_t st = stream_;
+  GPUGA_HIP(hipMemcpyAsync(pl.genes.get(), genes.data(), sizeof(float) * genes.size(),
+                           hipMemcpyHostToDevice, st));
+  launch_population(pl, st); // warm-up
+  GPUGA_HIP(hipEventRecord(ev0_, st));
+  for (int r = 0; r < reps; ++r) launch_population(pl, st);
+  GPUGA_HIP(hipEventRecord(ev1_, st));
+  GPUGA_HIP(hipStreamSynchronize(st));
+  float ms = 0.0f;
+  GPUGA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+  return ms / reps;
+}
+
+int Fitness::autotune(int P)
+{
+  // Candidate work-item sizes: the whole-configuration cost spread over
+  // 1/2 .. 1/32 of the resident waves.
+  long long total = 0, maxcfg = 0;
+  for (int c = 0; c < ts_.Nc; ++c) {
+    long long t = 0;
+    for (int i = ts_.Na_sum[c]; i < ts_.Na_sum[c] + ts_.Na[c]; ++i)
+      t += atom_cost(spec_.type, nl_.NN[i], c < ts_.Nc_force);
+    total += t;
+    maxcfg = std::max(maxcfg, t);
+  }
+  const int K = (P + WAVE - 1) / WAVE;
+  const long long waves = (long long)num_cu_ * 4 * 2;
+  std::vector<int> cands;
+  for (int div : {1, 2, 4, 8, 16}) {
+    long long c = total * K / std::max<long long>(1, waves * div / 2);
+    c = std::max<long long>(c, 64);
+    c = std::min<long long>(c, std::max<long long>(maxcfg, 64));
+    cands.push_back((int)c);
+  }
+  cands.push_back((int)std::max<long long>(maxcfg, 64)); // whole configurations
+  std::sort(cands.begin(), cands.end());
+  cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
+  int best = cands.back();
+  double best_ms = 1e30;
+  for (int c : cands) {
+    const double ms = time_plan(P, c, 3);
+    if (opt_.verbose) std::printf("autotune P=%d chunk_cost=%d: %.4f ms\n", P, c, ms);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  tuned_[P] = best;
+  return best;
+}
+
+} // namespace gpuga

@@ -1,0 +1,137 @@
+// Fitness evaluator: the drop-in for the reference's class Fitness
+// (src/fitness.cuh:30-83) and its Potential plug-ins (src/potential.cuh:22-41).
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "trainset.h"
+
+namespace gpuga {
+
+struct FitnessOptions {
+  int backend = 0;        // 0: population kernel, 1: per-individual launches
+  int chunk_cost = 0;     // work-item cost target; 0 = autotune on first use
+  int use_graph = 1;      // replay each generation from a captured HIP graph
+  int nn2b_bf16 = 1;      // NN2B hidden layer 2 on BF16 MFMA (else FP32)
+  int nn2b_grid = 4096;   // NN2B tabulation points
+  int verbose = 0;
+  int blocks_per_cu = 0;  // 0 = from the occupancy calculator
+};
+
+// Device-resident copy of one training set and its geometry tables.
+struct DeviceSet {
+  DeviceBuffer<int> Na, Na_sum, type, nn, bond0, nl, rev, ptype, tri0;
+  DeviceBuffer<float4> bond, tri, ri_tab;
+  DeviceBuffer<float> pe_ref, virial_ref, force_ref;
+  DevSet view{};
+  size_t bytes() const;
+};
+
+// Abstract plug-in with the reference's three calls (src/potential.cuh:26-40);
+// pointers are device pointers laid out as the reference lays them out
+// (f [3][N], virial [6][N], pe [N]).
+class Potential {
+public:
+  virtual ~Potential() = default;
+  virtual void initialize(int N, int max_Na) = 0;
+  virtual void update_potential(const std::vector<float>& physical) = 0;
+  virtual void find_force(const DevSet& set, float* f, float* virial, float* pe,
+                          hipStream_t st) = 0;
+};
+
+std::unique_ptr<Potential> make_potential(const PotentialSpec& spec, const DeviceSet& ds);
+
+// One population plan (work items, scratch) for a population size P.
+struct PopPlan {
+  int P = 0, K = 0, n_items = 0, grid = 0, block = 256, chunk_cost = 0;
+  DeviceBuffer<WorkItem> items;
+  DeviceBuffer<int> cfg_nchunks, cfg_slot0;
+  DeviceBuffer<double> part, err, terms;
+  DeviceBuffer<float> bondf, genes, fitness, nn2b_table;
+  DeviceBuffer<unsigned> counters; // [Nc*K] cnt | [K] blk_done | [1] fetch
+  PinnedBuffer<float> h_genes, h_fitness;
+  PinnedBuffer<double> h_terms;
+  PopCtl ctl{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~PopPlan();
+};
+
+class Fitness {
+public:
+  Fitness(const std::string& input_dir, const FitnessOptions& opt = FitnessOptions());
+  Fitness(const PotentialSpec& spec, TrainSet ts, const FitnessOptions& opt = FitnessOptions());
+  ~Fitness();
+
+  // Fitness::compute (src/fitness.cu:316-333): genes [P][V] in [0,1] -> fitness[P].
+  void compute(int P, const float* genes, float* fitness, double* terms = nullptr);
+  // The per-individual-launch form (one find_force + reduction + sync each).
+  void compute_plain(int P, const float* genes, float* fitness, double* terms = nullptr);
+  // Fitness::predict (src/fitness.cu:347-390): writes force/energy/virial.out.
+  void predict(const std::string& dir, const float* elite);
+  // Per-atom outputs of one physical parameter set (host arrays).
+  void find_force(const float* physical, float* f, float* virial, float* pe);
+  // Launch the population generation asynchronously / wait for it.
+  void compute_async(int P, const float* genes);
+  void compute_wait(float* fitness, double* terms);
+  // Tune the work-item size for population size P.
+  int autotune(int P);
+  // Work-item size from the training set alone (no timing; deterministic).
+  int heuristic_chunk_cost(int P) const;
+  // Device bytes a handle for this training set and population would hold.
+  static size_t estimate_bytes(const PotentialSpec& s, const TrainSet& ts, int P,
+                               const FitnessOptions& opt);
+
+  int number_of_variables() const { return spec_.V; }
+  const PotentialSpec& spec() const { return spec_; }
+  const TrainSet& trainset() const { return ts_; }
+  const NeighborList& neighbors() const { return nl_; }
+  const Geometry& geometry() const { return geo_; }
+  hipStream_t stream() const { return stream_; }
+  size_t device_bytes() const;
+  size_t plan_bytes(int P) const;
+  // Accumulated population-kernel time (HIP events on the launch stream).
+  double kernel_ms() const { return kernel_ms_; }
+  long kernel_launches() const { return kernel_launches_; }
+  void reset_timers() { kernel_ms_ = 0; kernel_launches_ = 0; }
+  void set_timing(bool on) { timing_ = on; }
+  FitnessOptions& options() { return opt_; }
+
+private:
+  void setup();
+  PopPlan& plan(int P);
+  void build_plan(PopPlan& pl, int P, int chunk_cost);
+  void launch_population(PopPlan& pl, hipStream_t st);
+  double time_plan(int P, int chunk_cost, int reps);
+  FitnessNorm norm() const;
+  NN2BGrid nn2b_grid_desc() const;
+
+  FitnessOptions opt_;
+  PotentialSpec spec_;
+  TrainSet ts_;
+  NeighborList nl_;
+  Geometry geo_;
+  DeviceSet ds_;
+  std::unique_ptr<Potential> potential_;
+  std::map<int, std::unique_ptr<PopPlan>> plans_;
+  std::map<int, int> tuned_;
+  hipStream_t stream_ = nullptr;
+  int device_ = 0;
+  int num_cu_ = 0;
+  int pending_P_ = 0;
+  float dmin_ = 0.0f, dmax_ = 0.0f; // pair-distance range of the training set
+  // per-atom outputs and plain-path scratch
+  DeviceBuffer<float> d_f_, d_virial_, d_pe_, d_params_, d_fit1_, d_pmin_, d_pmax_;
+  DeviceBuffer<double> d_cerr_, d_terms1_;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  bool timing_ = false;
+  double kernel_ms_ = 0.0;
+  long kernel_launches_ = 0;
+};
+
+} // namespace gpuga

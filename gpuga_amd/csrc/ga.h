@@ -1,0 +1,85 @@
+// Genetic algorithm driver: the drop-in for the reference's class GA
+// (src/ga.cuh:20-58, src/ga.cu:27-316).  The random numbers reproduce the
+// libstdc++ <random> stream the reference draws from, with explicit
+// algorithms so that the sequence does not depend on the host library.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <functional>
+#include <string>
+#include <vector>
+
+namespace gpuga {
+
+class Fitness;
+
+// std::mt19937 (Matsumoto & Nishimura 1998, C++ [rand.eng.mers]).
+class MT19937 {
+public:
+  explicit MT19937(uint32_t seed = 5489u) { seed_with(seed); }
+  void seed_with(uint32_t seed);
+  uint32_t operator()();
+  // uniform_real_distribution<float>(0, 1) of libstdc++: generate_canonical
+  // with one 32-bit draw, (float)x / 2^32, results >= 1 replaced by the float
+  // just below 1.
+  float canonical();
+  // uniform_int_distribution<int>(a, b) of libstdc++ >= 11: Lemire's
+  // nearly-divisionless method on a 64-bit product (ACM TOMACS 29(1), 2019).
+  int uniform_int(int a, int b);
+  const uint32_t* state() const { return mt_; }
+  int index() const { return idx_; }
+  void set_state(const uint32_t* s, int idx);
+
+private:
+  uint32_t mt_[624];
+  int idx_;
+};
+
+struct GAParams {
+  int maximum_generation = 1000; // src/ga.cuh:29-34 defaults
+  int population_size = 200;
+  int parent_number = 100;
+  float mutation_rate = 0.2f;
+  uint32_t seed = 12345678u;     // the reference's deterministic seed (src/ga.cu:47-48)
+};
+
+// ga.in reader (src/ga.cu:214-316 with the tokenizer of src/read_file.cu:46-72).
+// Extension: an optional `seed` keyword.  Population and parent counts need
+// not be multiples of 10; the child count must be even.
+GAParams read_ga(const std::string& dir);
+void validate_ga(const GAParams& p);
+
+class GA {
+public:
+  GA(const GAParams& p, int V, const std::vector<float>& pmin, const std::vector<float>& pmax);
+  // one generation after fitness() has been filled: sort, crossover, mutation
+  void sort_population();
+  void crossover();
+  void mutation(int generation);
+  // the reference's ga.out line (src/ga.cu:141-147)
+  void write_line(FILE* fid, int generation) const;
+  // full run as GA::GA + GA::compute (src/ga.cu:27-92); fitness callback
+  // evaluates P genes rows into fitness.  Writes <dir>/ga.out when dir != "".
+  void run(const std::string& dir, const std::function<void(int, const float*, float*)>& eval,
+           int print_every = 10);
+
+  std::vector<float>& population() { return population_; }
+  std::vector<float>& fitness() { return fitness_; }
+  const std::vector<float>& population() const { return population_; }
+  MT19937& rng() { return rng_; }
+  int V() const { return V_; }
+  const GAParams& params() const { return p_; }
+
+private:
+  int get_a_parent();
+  GAParams p_;
+  int V_;
+  int child_number_;
+  std::vector<float> pmin_, pmax_;
+  std::vector<float> population_, population_copy_, fitness_, cumulative_;
+  std::vector<int> index_;
+  MT19937 rng_;
+};
+
+} // namespace gpuga

@@ -1,0 +1,267 @@
+// Pair potentials on CDNA: rigid ion (RI) and the NN2B neural pair potential.
+//
+//   RI    U, U'/r           src/ri.cu:77-110, pair loop :133-187
+//   NN2B  MLP U, U'         src/nn2b.cu:49-78, cutoff :80-92, loop :115-171
+//
+// Single-parameter-set kernels keep the reference's thread-per-atom pair loop
+// over the full neighbour list.  The RI population kernel reads, per bond,
+// the parameter-independent factors erfc(a r)/r, its radial derivative,
+// r^-6 and 1/r from a table built once per training set, so a lane only
+// evaluates the parameter-dependent exponentials.
+#include "kernels.h"
+#include "population.h"
+
+namespace gpuga {
+
+// --------------------------------------------------------------------- RI
+
+struct RIBond {
+  float dinv6, ecoul, gcoul, dinv; // r^-6, erfc(a r)/r, -(d/dr)(erfc(a r)/r), 1/r
+};
+
+__device__ inline float sel3(const float* a, int t) { return t == 0 ? a[0] : (t == 1 ? a[1] : a[2]); }
+
+// U(r) and U'(r)/r for a pair of type sum t (src/ri.cu:77-110).
+__device__ inline void ri_pair(const RIParams& q, int t, float d, const float4& tab, float& p2,
+                               float& f2)
+{
+  const float a = sel3(q.a, t), b = sel3(q.b, t), c = sel3(q.c, t), qq = sel3(q.qq, t);
+  const float ex = expf(-d * b);
+  float p = a * ex - c * tab.x;
+  float g = 6.0f * c * tab.x * tab.w - a * ex * b;
+  p += qq * (tab.y - q.v_rc - q.dv_rc * (d - q.cutoff));
+  g -= qq * (tab.z + q.dv_rc);
+  if (t == 1) {
+    const float x = d - q.r0;
+    const float fr = q.mR * expf(q.kR * x);
+    const float fa = q.mA * expf(q.kA * x);
+    p += fr - fa;
+    g += q.kR * fr - q.kA * fa;
+  }
+  p2 = p;
+  f2 = g * tab.w;
+}
+
+__global__ void ri_plain(DevSet s, RIParams q, const float4* __restrict__ tab,
+                         float* __restrict__ f, float* __restrict__ virial, float* __restrict__ pe)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.N) return;
+  const size_t N = s.N;
+  const int nn = s.nn[i], b0 = s.bond0[i];
+  float fx = 0, fy = 0, fz = 0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wyz = 0, wzx = 0, e = 0;
+  for (int u = 0; u < nn; ++u) {
+    const int k = b0 + u;
+    const float4 r = s.bond[k];
+    float p2, f2;
+    ri_pair(q, s.ptype[k], r.w, tab[k], p2, f2);
+    fx += r.x * f2;
+    fy += r.y * f2;
+    fz += r.z * f2;
+    wxx -= r.x * r.x * f2 * 0.5f;
+    wyy -= r.y * r.y * f2 * 0.5f;
+    wzz -= r.z * r.z * f2 * 0.5f;
+    wxy -= r.x * r.y * f2 * 0.5f;
+    wyz -= r.y * r.z * f2 * 0.5f;
+    wzx -= r.z * r.x * f2 * 0.5f;
+    e += p2 * 0.5f;
+  }
+  f[i] = fx;
+  f[i + N] = fy;
+  f[i + 2 * N] = fz;
+  virial[i] = wxx;
+  virial[i + N] = wyy;
+  virial[i + 2 * N] = wzz;
+  virial[i + 3 * N] = wxy;
+  virial[i + 4 * N] = wyz;
+  virial[i + 5 * N] = wzx;
+  pe[i] = e;
+}
+
+struct RIPop {
+  static constexpr bool kForcePass = false;
+  using Lane = RIParams;
+  using Extra = const float4*;
+
+  __device__ static Lane make(const PopCtl& ctl, int p)
+  {
+    float x[15];
+#pragma unroll
+    for (int v = 0; v < 15; ++v)
+      x[v] = denormalise(ctl.pmin[v], ctl.pmax[v], ctl.genes[(size_t)p * ctl.V + v]);
+    return ri_params(x);
+  }
+
+  // Energy/virial configurations visit each unordered pair once (bonds with
+  // j > i, whose reverse carries the same |r| and -r); force configurations
+  // visit every bond of an atom so that F_i is complete in the lane.
+  __device__ static void chunk(const DevSet& s, const RIParams& q, const WorkItem& it,
+                               const PopCtl&, const float4* __restrict__ tab, Acc& acc)
+  {
+    const size_t N = s.N;
+    if (it.c < s.Nc_force) {
+      for (int i = it.a0; i < it.a1; ++i) {
+        const int nn = s.nn[i], b0 = s.bond0[i];
+        float fx = 0, fy = 0, fz = 0;
+        for (int u = 0; u < nn; ++u) {
+          const int b = b0 + u;
+          const float4 r = s.bond[b];
+          float p2, f2;
+          ri_pair(q, s.ptype[b], r.w, tab[b], p2, f2);
+          fx += r.x * f2;
+          fy += r.y * f2;
+          fz += r.z * f2;
+        }
+        const float dx = fx - s.force_ref[i], dy = fy - s.force_ref[i + N],
+                    dz = fz - s.force_ref[i + 2 * N];
+        acc.v[7] += (double)dx * dx + (double)dy * dy + (double)dz * dz;
+      }
+    } else {
+      for (int i = it.a0; i < it.a1; ++i) {
+        const int nn = s.nn[i], b0 = s.bond0[i];
+        float e = 0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wyz = 0, wzx = 0;
+        for (int u = 0; u < nn; ++u) {
+          const int b = b0 + u;
+          if (s.nl[b] < i) continue;
+          const float4 r = s.bond[b];
+          float p2, f2;
+          ri_pair(q, s.ptype[b], r.w, tab[b], p2, f2);
+          const float rx = r.x * f2, ry = r.y * f2, rz = r.z * f2;
+          wxx -= r.x * rx;
+          wyy -= r.y * ry;
+          wzz -= r.z * rz;
+          wxy -= r.x * ry;
+          wyz -= r.y * rz;
+          wzx -= r.z * rx;
+          e += p2;
+        }
+        acc.v[0] += e;
+        acc.v[1] += wxx;
+        acc.v[2] += wyy;
+        acc.v[3] += wzz;
+        acc.v[4] += wxy;
+        acc.v[5] += wyz;
+        acc.v[6] += wzx;
+      }
+    }
+  }
+
+  __device__ static double force_pass(const DevSet&, const PopCtl&, int, int, int) { return 0.0; }
+};
+
+void launch_ri_plain(const DevSet& s, const RIParams& q, const float4* tab, float* f,
+                         float* virial, float* pe, hipStream_t st)
+{
+  const int block = 128, grid = (s.N + block - 1) / block;
+  ri_plain<<<grid, block, 0, st>>>(s, q, tab, f, virial, pe);
+  GPUGA_HIP(hipGetLastError());
+}
+
+void launch_ri_population(const PopCtl& ctl, const DevSet& s, const float4* ri_tab, int grid,
+                          int block, hipStream_t st)
+{
+  population_kernel<RIPop><<<grid, block, 0, st>>>(ctl, s, ri_tab);
+  GPUGA_HIP(hipGetLastError());
+}
+
+int ri_population_occupancy(int block)
+{
+  int n = 0;
+  GPUGA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, population_kernel<RIPop>, block, 0));
+  return n;
+}
+
+// -------------------------------------------------------------------- NN2B
+
+constexpr int NN2B_MAX_PLAIN = 64;
+
+// One thread per atom, weights staged in LDS; the MLP is evaluated per bond as
+// the reference does (src/nn2b.cu:49-78), for any n <= 64.
+__global__ void nn2b_plain(DevSet s, int n, const float* __restrict__ w, float* __restrict__ f,
+                           float* __restrict__ virial, float* __restrict__ pe)
+{
+  extern __shared__ float sw[];
+  const int V = n * (n + 4) + 1;
+  for (int v = threadIdx.x; v < V; v += blockDim.x) sw[v] = w[v];
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.N) return;
+  const float* w0 = sw;
+  const float* b0 = sw + n;
+  const float* w1 = sw + 2 * n;
+  const float* b1 = sw + n * (n + 2);
+  const float* w2 = sw + n * (n + 3);
+  const float bias2 = sw[n * (n + 4)];
+  const size_t N = s.N;
+  const int nn = s.nn[i], k0 = s.bond0[i];
+  float fx = 0, fy = 0, fz = 0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wyz = 0, wzx = 0, e = 0;
+  float x1[NN2B_MAX_PLAIN], y1[NN2B_MAX_PLAIN];
+  for (int u = 0; u < nn; ++u) {
+    const float4 r = s.bond[k0 + u];
+    const float d = r.w;
+    for (int m = 0; m < n; ++m) {
+      x1[m] = tanhf(w0[m] * d - b0[m]);
+      y1[m] = (1.0f - x1[m] * x1[m]) * w0[m];
+    }
+    float p = 0.0f, g = 0.0f;
+    for (int a = 0; a < n; ++a) {
+      float z = 0.0f, y = 0.0f;
+      for (int m = 0; m < n; ++m) {
+        z += w1[a * n + m] * x1[m];
+        y += w1[a * n + m] * y1[m];
+      }
+      const float x2 = tanhf(z - b1[a]);
+      y *= 1.0f - x2 * x2;
+      p += w2[a] * x2;
+      g += w2[a] * y;
+    }
+    p -= bias2;
+    float fc, fcp;
+    if (d < NN2B_R1) {
+      fc = 1.0f;
+      fcp = 0.0f;
+    } else if (d < NN2B_R2) {
+      float sn, cs;
+      sincosf(NN2B_PI_FACTOR * (d - NN2B_R1), &sn, &cs);
+      fc = 0.5f * cs + 0.5f;
+      fcp = -sn * NN2B_PI_FACTOR * 0.5f;
+    } else {
+      fc = 0.0f;
+      fcp = 0.0f;
+    }
+    const float p2 = p * fc;
+    const float f2 = (g * fc + p2 * fcp) / d; // src/nn2b.cu:147-148
+    fx += r.x * f2;
+    fy += r.y * f2;
+    fz += r.z * f2;
+    wxx -= r.x * r.x * f2 * 0.5f;
+    wyy -= r.y * r.y * f2 * 0.5f;
+    wzz -= r.z * r.z * f2 * 0.5f;
+    wxy -= r.x * r.y * f2 * 0.5f;
+    wyz -= r.y * r.z * f2 * 0.5f;
+    wzx -= r.z * r.x * f2 * 0.5f;
+    e += p2 * 0.5f;
+  }
+  f[i] = fx;
+  f[i + N] = fy;
+  f[i + 2 * N] = fz;
+  virial[i] = wxx;
+  virial[i + N] = wyy;
+  virial[i + 2 * N] = wzz;
+  virial[i + 3 * N] = wxy;
+  virial[i + 4 * N] = wyz;
+  virial[i + 5 * N] = wzx;
+  pe[i] = e;
+}
+
+void launch_nn2b_plain(const DevSet& s, int n, const float* w, float* f, float* virial,
+                       float* pe, hipStream_t st)
+{
+  if (n > NN2B_MAX_PLAIN) fail(strfmt("NN2B supports at most %d neurons", NN2B_MAX_PLAIN));
+  const int block = 128, grid = (s.N + block - 1) / block;
+  const size_t lds = sizeof(float) * (n * (n + 4) + 1);
+  nn2b_plain<<<grid, block, lds, st>>>(s, n, w, f, virial, pe);
+  GPUGA_HIP(hipGetLastError());
+}
+
+} // namespace gpuga

@@ -1,0 +1,196 @@
+// Serving loop (host).
+#include "serve.h"
+
+#include <algorithm>
+#include <cstring>
+#include <thread>
+
+namespace gpuga {
+
+Server::Server(double budget_bytes, int max_active, const FitnessOptions& opt)
+  : budget_(budget_bytes), max_active_(std::max(1, max_active)), opt_(opt)
+{
+}
+
+Server::~Server()
+{
+  for (auto& j : jobs_)
+    if (j->gaout) std::fclose(j->gaout);
+}
+
+int Server::submit(const std::string& dir, bool write_outputs)
+{
+  auto j = std::make_unique<Job>();
+  j->id = (int)jobs_.size();
+  j->dir = dir;
+  j->write = write_outputs;
+  try {
+    j->spec = read_potential(dir);
+    j->ts = read_train(dir);
+    j->gp = read_ga(dir);
+    j->est_bytes = Fitness::estimate_bytes(j->spec, j->ts, j->gp.population_size, opt_);
+    queue_.push_back(j->id);
+  } catch (const std::exception& e) {
+    j->state = 3;
+    j->error = e.what();
+  }
+  jobs_.push_back(std::move(j));
+  return jobs_.back()->id;
+}
+
+int Server::active() const
+{
+  int n = 0;
+  for (auto& j : jobs_) n += j->state == 1;
+  return n;
+}
+
+int Server::finished() const
+{
+  int n = 0;
+  for (auto& j : jobs_) n += j->state >= 2;
+  return n;
+}
+
+void Server::admit(Job& j)
+{
+  try {
+    FitnessOptions o = opt_;
+    j.fit = std::make_unique<Fitness>(j.spec, std::move(j.ts), o);
+    // deterministic work-item size: the same chunking alone or co-scheduled
+    if (o.chunk_cost <= 0) j.fit->options().chunk_cost = j.fit->heuristic_chunk_cost(j.gp.population_size);
+    j.ga = std::make_unique<GA>(j.gp, j.fit->number_of_variables(), j.fit->spec().pmin,
+                                j.fit->spec().pmax);
+    if (j.write) {
+      const std::string path = j.dir + "/ga.out";
+      j.gaout = std::fopen(path.c_str(), "w");
+      if (!j.gaout) fail(strfmt("Failed to open %s!", path.c_str()));
+    }
+    j.state = 1;
+    used_ += (double)j.est_bytes;
+  } catch (const std::exception& e) {
+    retire(j, false, e.what());
+  }
+}
+
+void Server::launch(Job& j)
+{
+  j.t_launch = std::chrono::steady_clock::now();
+  j.fit->compute_async(j.gp.population_size, j.ga->population().data());
+  j.inflight = true;
+}
+
+void Server::complete(Job& j)
+{
+  // GA::compute body for one generation (src/ga.cu:82-88)
+  j.fit->compute_wait(j.ga->fitness().data(), nullptr);
+  j.inflight = false;
+  const auto now = std::chrono::steady_clock::now();
+  lat_ms_.push_back(std::chrono::duration<double, std::milli>(now - j.t_launch).count());
+  j.ga->sort_population();
+  if (j.gaout) {
+    j.ga->write_line(j.gaout, j.gen);
+    std::fflush(j.gaout);
+  }
+  j.ga->crossover();
+  j.ga->mutation(j.gen);
+  ++j.gen;
+  if (j.gen >= j.gp.maximum_generation) retire(j, true);
+}
+
+void Server::retire(Job& j, bool ok, const std::string& err)
+{
+  if (ok) {
+    j.best = j.ga->fitness()[0];
+    const int V = j.ga->V();
+    j.elite.resize(V);
+    for (int m = 0; m < V; ++m) {
+      const float a = j.fit->spec().pmin[m];
+      const float b = j.fit->spec().pmax[m] - a;
+      const float bx = b * j.ga->population()[m];
+      j.elite[m] = a + bx;
+    }
+    if (j.write) {
+      try {
+        j.fit->predict(j.dir, j.ga->population().data());
+      } catch (const std::exception& e) {
+        ok = false;
+        j.error = e.what();
+      }
+    }
+  } else {
+    j.error = err;
+  }
+  if (j.gaout) {
+    std::fclose(j.gaout);
+    j.gaout = nullptr;
+  }
+  if (j.state == 1) used_ -= (double)j.est_bytes;
+  j.state = ok ? 2 : 3;
+  j.fit.reset();
+  j.ga.reset();
+}
+
+void Server::poll()
+{
+  // 1. consume finished generations
+  for (auto& jp : jobs_) {
+    Job& j = *jp;
+    if (j.state != 1 || !j.inflight) continue;
+    const hipError_t q = hipStreamQuery(j.fit->stream());
+    if (q == hipErrorNotReady) continue;
+    if (q != hipSuccess) {
+      retire(j, false, hipGetErrorString(q));
+      continue;
+    }
+    try {
+      complete(j);
+    } catch (const std::exception& e) {
+      retire(j, false, e.what());
+    }
+  }
+  // 2. admit queued jobs while a slot and the memory budget allow
+  while (!queue_.empty() && active() < max_active_) {
+    Job& j = *jobs_[queue_.front()];
+    if (active() > 0 && used_ + (double)j.est_bytes > budget_) break;
+    queue_.pop_front();
+    admit(j);
+  }
+  // 3. start the next generation of every idle active job
+  for (auto& jp : jobs_) {
+    Job& j = *jp;
+    if (j.state == 1 && !j.inflight) {
+      try {
+        launch(j);
+      } catch (const std::exception& e) {
+        retire(j, false, e.what());
+      }
+    }
+  }
+}
+
+void Server::drain()
+{
+  while (finished() < (int)jobs_.size()) {
+    const size_t before = lat_ms_.size();
+    poll();
+    if (lat_ms_.size() == before) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+void Server::job_info(int id, int* state, int* gens, float* best, float* elite) const
+{
+  if (id < 0 || id >= (int)jobs_.size()) fail(strfmt("no job %d", id));
+  const Job& j = *jobs_[id];
+  if (j.state == 3 && !j.error.empty() && state) {
+    *state = 3;
+  } else if (state) {
+    *state = j.state;
+  }
+  if (gens) *gens = j.gen;
+  if (best) *best = j.best;
+  if (elite && !j.elite.empty()) std::memcpy(elite, j.elite.data(), sizeof(float) * j.elite.size());
+  if (j.state == 3) fail(strfmt("job %d failed: %s", id, j.error.c_str()));
+}
+
+} // namespace gpuga

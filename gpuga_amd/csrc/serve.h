@@ -1,0 +1,72 @@
+// Serving loop: concurrent GA fitting jobs with resident training sets.
+//
+// The reference runs the job directories of input.txt one after another
+// (src/main.cu:28-50).  Here jobs are queued, admitted while their device
+// bytes fit a budget (and a slot is free), and each admitted job advances one
+// generation at a time on its own HIP stream, so generations of different
+// jobs overlap on the device.  A finished job releases its memory and the
+// next queued job is admitted (continuous batching).  The work-item size of
+// each job is a deterministic function of its training set, so a job's
+// results are the same whether it runs alone or with others.
+#pragma once
+
+#include <chrono>
+#include <cstdio>
+#include <deque>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fitness.h"
+#include "ga.h"
+
+namespace gpuga {
+
+class Server {
+public:
+  Server(double budget_bytes, int max_active, const FitnessOptions& opt);
+  ~Server();
+  int submit(const std::string& dir, bool write_outputs);
+  void poll();
+  void drain();
+  int active() const;
+  int queued() const { return (int)queue_.size(); }
+  int finished() const;
+  void job_info(int id, int* state, int* gens, float* best, float* elite) const;
+  const std::vector<double>& latencies() const { return lat_ms_; }
+
+private:
+  struct Job {
+    int id = 0;
+    std::string dir;
+    bool write = false;
+    int state = 0; // 0 queued, 1 active, 2 done, 3 failed
+    std::string error;
+    PotentialSpec spec;
+    TrainSet ts;
+    GAParams gp;
+    size_t est_bytes = 0;
+    std::unique_ptr<Fitness> fit;
+    std::unique_ptr<GA> ga;
+    FILE* gaout = nullptr;
+    int gen = 0;
+    bool inflight = false;
+    std::chrono::steady_clock::time_point t_launch;
+    float best = 0.0f;
+    std::vector<float> elite;
+  };
+  void admit(Job& j);
+  void launch(Job& j);
+  void complete(Job& j);
+  void retire(Job& j, bool ok, const std::string& err = "");
+
+  double budget_;
+  int max_active_;
+  FitnessOptions opt_;
+  std::vector<std::unique_ptr<Job>> jobs_;
+  std::deque<int> queue_;
+  double used_ = 0.0;
+  std::vector<double> lat_ms_;
+};
+
+} // namespace gpuga

@@ -1,0 +1,80 @@
+// Training set, potential description and parameter-independent geometry.
+//
+// Host-side mirror of the reference's set-up layer:
+//   potential.in reader        src/fitness.cu:220-314
+//   train.in reader            src/fitness.cu:59-218
+//   box inverse                src/fitness.cu:40-57
+//   neighbour list             src/neighbor.cu:30-83, src/mic.cuh:20-32
+// plus the tables the population kernels read instead of recomputing the
+// minimum-image geometry per individual (the reference recomputes it in every
+// kernel, src/minimal_tersoff.cu:150-153 and :162-167).
+#pragma once
+
+#include <string>
+#include <vector>
+
+namespace gpuga {
+
+enum PotentialType { POT_RI = 0, POT_TERSOFF1 = 1, POT_TERSOFF2 = 2, POT_NN2B = 3 };
+
+struct PotentialSpec {
+  int type = POT_TERSOFF1;
+  int num_neurons = 0;
+  int V = 0;                 // number_of_variables
+  float cutoff = 0.0f;       // neighbour-list cutoff
+  float w_force = 0.0f, w_energy = 0.0f, w_stress = 0.0f;
+  std::vector<float> pmin, pmax;
+  std::vector<std::string> names;
+};
+
+struct TrainSet {
+  int Nc = 0, Nc_force = 0, N = 0, N_force = 0, max_Na = 0;
+  std::vector<int> Na, Na_sum, type;
+  std::vector<float> h;          // [Nc][18]: box columns a, b, c, then inverse
+  std::vector<float> r;          // [3][N]
+  std::vector<float> pe_ref;     // [Nc]
+  std::vector<float> virial_ref; // [6][Nc]
+  std::vector<float> force_ref;  // [3][N]
+  // Normalisers of the three relative RMS errors, accumulated in FP64.
+  double energy_minimum = 0, force_square_sum = 0, potential_square_sum = 0,
+         virial_square_sum = 0;
+};
+
+// CSR neighbour list: atom i's neighbours are NL[start[i] .. start[i]+NN[i]),
+// ascending; rev[b] is the bond index of the reverse bond (the slot of i in
+// j's list, or j's first slot if absent, as src/minimal_tersoff.cu:368-375).
+struct NeighborList {
+  std::vector<int> NN, start, NL, rev;
+  int total = 0;
+  int max_nn = 0;
+};
+
+struct Vec4 {
+  float x, y, z, w;
+};
+
+// Parameter-independent geometry, computed once per training set in FP64 from
+// the FP32 inputs and stored in FP32.
+struct Geometry {
+  std::vector<Vec4> bond;     // per bond: r_ij (x, y, z) and |r_ij|
+  std::vector<int> tri_start; // per atom: offset of its NN x NN triplet block
+  std::vector<Vec4> tri;      // per (i, s, t): cos(theta_jik), d cos / d r_ij
+  long long tri_total = 0;
+};
+
+PotentialSpec read_potential(const std::string& dir);
+TrainSet read_train(const std::string& dir);
+// Builds a training set from arrays (box9 = file-order box rows per config).
+TrainSet make_train(int Nc, int Nc_force, const int* Na, const int* type, const float* box9,
+                    const float* r, const float* pe_ref, const float* virial_ref,
+                    const float* force_ref);
+void box_inverse(float* h18);
+void finish_normalisers(TrainSet& ts);
+
+NeighborList build_neighbors(const TrainSet& ts, float cutoff);
+Geometry build_geometry(const TrainSet& ts, const NeighborList& nl, bool triplets);
+
+// Index of the configuration that holds atom i.
+int config_of(const TrainSet& ts, int i);
+
+} // namespace gpuga

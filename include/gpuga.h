@@ -1,0 +1,152 @@
+/*
+ * gpuga — C ABI of the MI355X fitness-evaluation library.
+ *
+ * Every entry point returns 0 on success and non-zero on failure, with the
+ * message available from gpuga_last_error() (per calling thread).  Arrays are
+ * plain host pointers; no entry point keeps a pointer it was given.  One
+ * handle must not be used by two host threads at the same time.
+ *
+ * Each declaration names the reference interface it replaces
+ * (brucefan1983/GPUGA, file:line).
+ */
+#ifndef GPUGA_H
+#define GPUGA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gpuga_fitness gpuga_fitness;
+typedef struct gpuga_ga gpuga_ga;
+typedef struct gpuga_server gpuga_server;
+
+typedef struct gpuga_options {
+  int backend;    /* 0: one population launch per generation (default);
+                     1: per-individual launches, as src/fitness.cu:318-332 */
+  int chunk_cost; /* work-item size of the population launch; 0 = autotune */
+  int use_graph;  /* 1: replay the population launch from a HIP graph */
+  int nn2b_bf16;  /* 1: NN2B hidden layer on BF16 MFMA; 0: FP32 */
+  int nn2b_grid;  /* NN2B tabulation points (default 1024) */
+  int verbose;
+} gpuga_options;
+
+/* Thread-local text of the last failure (src/error.cu:21-25 prints it). */
+const char* gpuga_last_error(void);
+void gpuga_default_options(gpuga_options* opt);
+
+/* Fitness::Fitness(char* input_dir), src/fitness.cu:30-38: reads
+ * potential.in and train.in, builds the neighbour list, keeps the training
+ * set resident on the current HIP device. */
+int gpuga_fitness_create(const char* input_dir, const gpuga_options* opt, gpuga_fitness** out);
+
+/* The same from arrays in train.in order: Na[Nc], type[N], box9[Nc][9] (file
+ * rows a, b, c), r[3][N], pe_ref[Nc], virial_ref[6][Nc], force_ref[3][N];
+ * pmin/pmax[V] as potential.in (ignored for NN2B: [-3, 3]), weights as
+ * potential.in lines 3-5 (src/fitness.cu:267-298). */
+int gpuga_fitness_create_arrays(int potential_type, int num_neurons, float cutoff,
+                                float weight_force, float weight_energy, float weight_virial,
+                                const float* pmin, const float* pmax, int Nc, int Nc_force,
+                                const int* Na, const int* type, const float* box9, const float* r,
+                                const float* pe_ref, const float* virial_ref,
+                                const float* force_ref, const gpuga_options* opt,
+                                gpuga_fitness** out);
+
+void gpuga_fitness_destroy(gpuga_fitness* h);
+
+/* Fitness::number_of_variables and parameters_min/max, src/fitness.cuh:35-38 */
+int gpuga_fitness_num_variables(const gpuga_fitness* h);
+int gpuga_fitness_bounds(const gpuga_fitness* h, float* pmin, float* pmax);
+
+/* Sizes: out[0..9] = potential_type, Nc, Nc_force, N, N_force, max_Na,
+ * bonds, max neighbours, V, num_neurons (src/fitness.cuh:55-63). */
+int gpuga_fitness_info(const gpuga_fitness* h, int* out10);
+
+/* Fitness::compute(int P, float* genes, float* fitness), src/fitness.cu:316-333.
+ * genes: [P][V] in [0,1], row-major; fitness: [P].  Blocking. */
+int gpuga_fitness_compute(gpuga_fitness* h, int P, const float* genes, float* fitness);
+
+/* As above, also returning terms[P][3] = relative RMS error of energy,
+ * virial, force (get_fitness_energy/stress/force, src/fitness.cu:437-555). */
+int gpuga_fitness_compute_terms(gpuga_fitness* h, int P, const float* genes, float* fitness,
+                                double* terms);
+
+/* The per-individual-launch form of Fitness::compute (src/fitness.cu:318-332):
+ * one find_force, one reduction and one host synchronisation per individual. */
+int gpuga_fitness_compute_plain(gpuga_fitness* h, int P, const float* genes, float* fitness,
+                                double* terms);
+
+/* Fitness::predict(char* dir, float* elite), src/fitness.cu:347-390: writes
+ * force.out, energy.out, virial.out into dir. */
+int gpuga_fitness_predict(gpuga_fitness* h, const char* dir, const float* elite_genes);
+
+/* Potential::update_potential + Potential::find_force, src/potential.cuh:26-40,
+ * for physical parameters params[V]: f[3][N], virial[6][N], pe[N]
+ * (any output may be NULL). */
+int gpuga_fitness_find_force(gpuga_fitness* h, const float* params, float* f, float* virial,
+                             float* pe);
+
+/* Neighbour list of Neighbor::compute, src/neighbor.cu:30-83, in CSR form:
+ * NN[N], start[N], NL[bonds] (ascending j per atom). */
+int gpuga_fitness_neighbors(const gpuga_fitness* h, int* NN, int* start, int* NL);
+
+/* Host-only set-up check (no HIP device used): reads potential.in and
+ * train.in from dir and builds the neighbour list of Neighbor::compute
+ * (src/neighbor.cu:30-66).  sizes[0..3] = N, bonds, max neighbours, Nc; call
+ * once with NULL arrays for the sizes, then with NN[N], start[N], NL[bonds]. */
+int gpuga_inspect_neighbors(const char* input_dir, int* sizes4, int* NN, int* start, int* NL);
+
+/* Work-item size chosen by timing candidate sizes for population P. */
+int gpuga_fitness_autotune(gpuga_fitness* h, int P, int* chunk_cost);
+
+/* HIP-event timing of the population launches (or of each individual's
+ * launches on the per-individual backend) on the library's stream. */
+int gpuga_fitness_set_timing(gpuga_fitness* h, int on);
+int gpuga_fitness_timing(gpuga_fitness* h, double* kernel_ms, long* launches, int reset);
+
+/* Device bytes held by the handle (training set, tables, plans). */
+int gpuga_fitness_device_bytes(const gpuga_fitness* h, double* bytes);
+
+/* ---------------------------------------------------------------- GA
+ * class GA, src/ga.cu:27-212, with the random stream of std::mt19937 and
+ * libstdc++'s distributions (explicit algorithms). */
+int gpuga_ga_create(int population_size, int parent_number, int V, int maximum_generation,
+                    float mutation_rate, unsigned seed, gpuga_ga** out);
+void gpuga_ga_destroy(gpuga_ga* g);
+/* genes [P][V] (copy out / in) */
+int gpuga_ga_get_population(const gpuga_ga* g, float* genes);
+int gpuga_ga_set_population(gpuga_ga* g, const float* genes);
+/* After fitness[P] of the current population: sort_population, crossover,
+ * mutation(generation) (src/ga.cu:83-88).  sorted_fitness[P] may be NULL. */
+int gpuga_ga_step(gpuga_ga* g, int generation, const float* fitness, float* sorted_fitness);
+
+/* GPUGA::GPUGA(char* dir), src/gpuga.cu:24-28: reads ga.in, potential.in,
+ * train.in from dir, runs the GA, writes ga.out, force.out, energy.out,
+ * virial.out.  seconds_per_generation may be NULL. */
+int gpuga_run(const char* input_dir, const gpuga_options* opt, double* seconds_per_generation);
+
+/* ----------------------------------------------------------- serving
+ * Concurrent fitting jobs (no reference counterpart: the reference runs the
+ * directories of input.txt one after another, src/main.cu:28-50).  Jobs are
+ * admitted while their device bytes fit the budget; each admitted job keeps
+ * its training set and population resident and advances one generation at a
+ * time on its own stream, so generations of different jobs overlap. */
+int gpuga_server_create(double memory_budget_bytes, int max_active, const gpuga_options* opt,
+                        gpuga_server** out);
+void gpuga_server_destroy(gpuga_server* s);
+/* Queue a job directory (ga.in, potential.in, train.in); returns its id. */
+int gpuga_server_submit(gpuga_server* s, const char* input_dir, int write_outputs, int* job_id);
+/* Run the scheduler until every submitted job has finished. */
+int gpuga_server_drain(gpuga_server* s);
+/* Advance the scheduler by at most one generation per active job. */
+int gpuga_server_poll(gpuga_server* s, int* active, int* queued, int* finished);
+/* Job results: state (0 queued, 1 active, 2 done, 3 failed), generations done,
+ * best fitness, elite physical parameters[V] (may be NULL). */
+int gpuga_server_job(const gpuga_server* s, int job_id, int* state, int* generations,
+                     float* best_fitness, float* elite);
+/* Per-generation latencies (ms) of all jobs in completion order; n in/out. */
+int gpuga_server_latencies(const gpuga_server* s, double* ms, int* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,143 @@
+"""GPU parity: the HIP library (through its C ABI) against the FP64 oracle.
+
+Tolerances (FP32 pair/triplet arithmetic, FP64 per-configuration sums):
+  fitness and its three terms     1e-5 relative     (Tersoff, RI, NN2B-FP32)
+  NN2B with BF16 MFMA hidden layer 3e-2 relative    (stated BF16 error)
+  per-atom force / virial / energy 2e-5 of the case's largest magnitude
+                                   (per-atom FP32 accumulation of bond terms)
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+@pytest.fixture(scope="module")
+def G():
+    import gpuga_amd
+    return gpuga_amd
+
+
+_FIT = {}
+
+
+def product(G, golden, name, **opts):
+    key = (name, tuple(sorted(opts.items())))
+    if key not in _FIT:
+        _FIT[key] = G.Fitness(golden(name), **opts)
+    return _FIT[key]
+
+
+def genes_for(V, P, seed):
+    return np.random.default_rng(seed).random((P, V), dtype=np.float32)
+
+
+@pytest.mark.parametrize("name", ["si_diamond", "gaas", "si_type2", "nn2b", "hfo2"])
+def test_find_force_matches_oracle(G, golden, oracle_case, name):
+    print_case("find_force", name)
+    orc = oracle_case(name)
+    fit = product(G, golden, name)
+    pmin, pmax = orc.bounds()
+    g = genes_for(orc.V, 1, 3)[0]
+    params = orc.denormalise(g)
+    f, w, e = fit.find_force(params)
+    fo, wo, eo = orc.find_force(params)
+    for got, ref, what in ((f, fo, "force"), (w, wo, "virial"), (e, eo, "energy")):
+        scale = max(np.abs(ref).max(), 1e-3)
+        err = np.abs(got.astype(np.float64) - ref).max() / scale
+        assert err < 2e-5, f"{name} {what}: max error {err:.3e} of scale {scale:.3e}"
+
+
+@pytest.mark.parametrize("name,P", [("si_diamond", 70), ("gaas", 20), ("si_type2", 20),
+                                    ("nn2b", 64)])
+def test_population_fitness_matches_oracle(G, golden, oracle_case, name, P):
+    print_case("population", name, P)
+    orc = oracle_case(name)
+    fit = product(G, golden, name, nn2b_bf16=0)
+    genes = genes_for(orc.V, P, 7)
+    got, terms = fit.compute(genes, terms=True)
+    ref, rterms = orc.fitness(genes, threads=THREADS)
+    rel = np.abs(got - ref) / np.abs(ref)
+    assert rel.max() < 1e-5, f"{name}: fitness rel error {rel.max():.3e}"
+    trel = np.abs(terms - rterms) / np.maximum(np.abs(rterms), 1e-30)
+    assert trel.max() < 1e-5, f"{name}: term rel error {trel.max():.3e}"
+
+
+def test_population_rigid_ion_matches_oracle(G, golden, oracle_case):
+    print_case("population", "hfo2")
+    orc = oracle_case("hfo2")
+    fit = product(G, golden, "hfo2")
+    genes = genes_for(orc.V, 3, 9)
+    got, terms = fit.compute(genes, terms=True)
+    ref, rterms = orc.fitness(genes, threads=THREADS)
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-5
+    np.testing.assert_allclose(terms, rterms, rtol=1e-5)
+
+
+def test_nn2b_bf16_error_is_stated_bound(G, golden, oracle_case):
+    print_case("population", "nn2b bf16")
+    orc = oracle_case("nn2b")
+    fit = product(G, golden, "nn2b", nn2b_bf16=1)
+    genes = genes_for(orc.V, 64, 5)
+    got = fit.compute(genes)
+    ref, _ = orc.fitness(genes, threads=THREADS)
+    rel = np.abs(got - ref) / np.abs(ref)
+    print("nn2b bf16 fitness rel error: max %.3e median %.3e" % (rel.max(), np.median(rel)))
+    assert rel.max() < 3e-2
+
+
+@pytest.mark.parametrize("name", ["si_diamond", "hfo2", "nn2b"])
+def test_population_equals_per_individual_path(G, golden, name):
+    print_case("plain-vs-population", name)
+    fit = product(G, golden, name, nn2b_bf16=0)
+    genes = genes_for(fit.V, 10, 13)
+    a, ta = fit.compute(genes, terms=True)
+    b, tb = fit.compute(genes, terms=True, plain=True)
+    np.testing.assert_allclose(ta, tb, rtol=2e-5)
+    np.testing.assert_allclose(a, b, rtol=2e-5)
+
+
+@pytest.mark.parametrize("P", [1, 63, 64, 65, 130])
+def test_population_sizes(G, golden, oracle_case, P):
+    print_case("population size", P)
+    orc = oracle_case("si_diamond")
+    fit = product(G, golden, "si_diamond")
+    genes = genes_for(orc.V, P, 100 + P)
+    got = fit.compute(genes)
+    sel = np.unique(np.linspace(0, P - 1, min(P, 6)).astype(int))
+    ref, _ = orc.fitness(genes[sel], threads=THREADS)
+    np.testing.assert_allclose(got[sel], ref, rtol=1e-5)
+
+
+def test_replay_is_bit_identical(G, golden):
+    print_case("graph replay")
+    fit = product(G, golden, "si_diamond")
+    genes = genes_for(fit.V, 200, 21)
+    first, t1 = fit.compute(genes, terms=True)
+    for _ in range(4):
+        again, t2 = fit.compute(genes, terms=True)
+        assert np.array_equal(first, again)
+        assert np.array_equal(t1, t2)
+    eager = G.Fitness(golden("si_diamond"), use_graph=0, chunk_cost=fit_chunk(fit, 200))
+    e, te = eager.compute(genes, terms=True)
+    np.testing.assert_allclose(te, t1, rtol=1e-12)
+
+
+def fit_chunk(fit, P):
+    return fit.autotune(P)
+
+
+def test_chunking_does_not_change_results(G, golden):
+    print_case("chunking")
+    genes = genes_for(9, 100, 33)
+    small = G.Fitness(golden("si_diamond"), chunk_cost=64)
+    whole = G.Fitness(golden("si_diamond"), chunk_cost=10 ** 8)
+    a, ta = small.compute(genes, terms=True)
+    b, tb = whole.compute(genes, terms=True)
+    np.testing.assert_allclose(ta, tb, rtol=1e-12)
