@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--workload", default="tersoff_batched", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain", action="store_true", help="per-individual launches")
+    ap.add_argument("--chunk-cost", type=int, default=0,
+                    help="work-item size; 0 = autotune before timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; multi-GPU runs are not part of this build "
@@ -95,9 +97,11 @@ def main():
     import gpuga_amd
     from gpuga_amd import synth
     d, w = make_workload(args.workload)
-    fit = gpuga_amd.Fitness(arrays=d, backend=1 if args.plain else 0)
+    fit = gpuga_amd.Fitness(arrays=d, backend=1 if args.plain else 0, chunk_cost=args.chunk_cost)
     P, V, N = w["P"], fit.V, fit.N
-    chunk = fit.autotune(P) if not args.plain else 0
+    chunk = args.chunk_cost
+    if not args.plain and chunk == 0:
+        chunk = fit.autotune(P)
     ga = gpuga_amd.GA(P, P // 2, V, maximum_generation=args.warmup + args.steps,
                       mutation_rate=0.2, seed=12345678)
 

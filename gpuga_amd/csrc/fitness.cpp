@@ -8,7 +8,7 @@
 
 namespace gpuga {
 
-int tersoff_population_occupancy(int block);
+int tersoff_population_occupancy(int cls, bool two, int block);
 int ri_population_occupancy(int block);
 int nn2b_population_occupancy(int block);
 
@@ -432,7 +432,7 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
     switch (spec_.type) {
     case POT_RI: occ = ri_population_occupancy(pl.block); break;
     case POT_NN2B: occ = nn2b_population_occupancy(pl.block); break;
-    default: occ = tersoff_population_occupancy(pl.block); break;
+    default: occ = tersoff_population_occupancy(nn_class(nl_.max_nn), spec_.type == POT_TERSOFF2, pl.block); break;
     }
     occ = std::max(occ, 1);
   }
@@ -472,7 +472,9 @@ void Fitness::launch_population(PopPlan& pl, hipStream_t st)
     launch_nn2b_population(pl.ctl, ds_.view, g, pl.nn2b_table.get(), pl.grid, pl.block, st);
     break;
   }
-  default: launch_tersoff_population(pl.ctl, ds_.view, pl.grid, pl.block, st); break;
+  default:
+    launch_tersoff_population(pl.ctl, ds_.view, nn_class(nl_.max_nn), pl.grid, pl.block, st);
+    break;
   }
 }
 

@@ -24,6 +24,7 @@
 #pragma once
 
 #include "device.h"
+#include "fastmath.h"
 
 namespace gpuga {
 
@@ -130,7 +131,7 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
       if (it == (unsigned)ctl.n_items + total_waves - 1u && lane == 0) *ctl.fetch = 0u;
       break;
     }
-    const WorkItem item = ctl.items[it];
+    const WorkItem item = ldu(ctl.items, it);
     const int k = item.k;
     // individual of this lane; lanes past P repeat the last one and write nothing
     const int p = min(k * WAVE + lane, ctl.P - 1);
@@ -142,10 +143,10 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
     Pot::chunk(set, q, item, ctl, ex, acc);
 
     const int c = item.c;
-    const int nch = ctl.cfg_nchunks[c];
+    const int nch = ldu(ctl.cfg_nchunks, c);
     const bool is_force = c < set.Nc_force;
     if (nch > 1) {
-      double* slot = ctl.part + ((size_t)(ctl.cfg_slot0[c] + item.chunk) * ctl.K + k) * 8 * WAVE;
+      double* slot = ctl.part + ((size_t)(ldu(ctl.cfg_slot0, c) + item.chunk) * ctl.K + k) * 8 * WAVE;
 #pragma unroll
       for (int u = 0; u < 8; ++u) slot[u * WAVE + lane] = acc.v[u];
       __threadfence();
@@ -157,7 +158,7 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
 #pragma unroll
       for (int u = 0; u < 8; ++u) acc.v[u] = 0.0;
       for (int ch = 0; ch < nch; ++ch) {
-        const double* s = ctl.part + ((size_t)(ctl.cfg_slot0[c] + ch) * ctl.K + k) * 8 * WAVE;
+        const double* s = ctl.part + ((size_t)(ldu(ctl.cfg_slot0, c) + ch) * ctl.K + k) * 8 * WAVE;
 #pragma unroll
         for (int u = 0; u < 8; ++u) acc.v[u] += s[u * WAVE + lane];
       }
@@ -166,11 +167,11 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
     if (is_force) {
       eF = Pot::kForcePass ? Pot::force_pass(set, ctl, c, k, lane) : acc.v[7];
     } else {
-      const double dE = acc.v[0] - (double)set.pe_ref[c];
+      const double dE = acc.v[0] - (double)ldu(set.pe_ref, c);
       eE = dE * dE;
 #pragma unroll
       for (int u = 0; u < 6; ++u) {
-        const double dW = acc.v[1 + u] - (double)set.virial_ref[c + (size_t)Nc * u];
+        const double dW = acc.v[1 + u] - (double)ldu(set.virial_ref, c + (long)Nc * u);
         eW += dW * dW;
       }
     }

@@ -10,6 +10,7 @@
 //
 // Formulas: src/minimal_tersoff.cu:62-119 (fR, fA, fc, g), :147-185 (bond
 // order), :250-307 (partial force f_ij), :355-392 (force and virial).
+#include "fastmath.h"
 #include "kernels.h"
 #include "population.h"
 
@@ -17,54 +18,85 @@ namespace gpuga {
 
 // ------------------------------------------------------------ scalar pieces
 
-__device__ inline void tersoff_fc(const TersoffParams& q, float d, float& fc, float& fcp)
+// Per-parameter-set constants in the form the kernels use: exponent
+// prefactors scaled by log2(e) for the exp2 instruction, 1/(R2-R1) and
+// 0.5 pi/(R2-R1) for the taper.
+struct TLane {
+  float cR[2], kR[2], kR2[2], cA[2], kA[2], kA2[2], r0[2];
+  float en, mhon, beta, beta2, h, r1, r2, inv_span, pif_half;
+};
+
+__host__ __device__ inline TLane tersoff_lane(const TersoffParams& p)
 {
-  if (d < q.r1) {
-    fc = 1.0f;
-    fcp = 0.0f;
-  } else if (d < q.r2) {
-    float s, c;
-    sincosf(q.pi_factor * (d - q.r1), &s, &c);
-    fc = 0.5f * c + 0.5f;
-    fcp = -s * q.pi_factor * 0.5f;
-  } else {
-    fc = 0.0f;
-    fcp = 0.0f;
+  TLane q;
+  for (int k = 0; k < 2; ++k) {
+    q.cR[k] = p.cR[k];
+    q.kR[k] = p.kR[k];
+    q.kR2[k] = p.kR[k] * LOG2E;
+    q.cA[k] = p.cA[k];
+    q.kA[k] = p.kA[k];
+    q.kA2[k] = p.kA[k] * LOG2E;
+    q.r0[k] = p.r0[k];
+  }
+  q.en = p.en;
+  q.mhon = p.minus_half_over_n;
+  q.beta = p.beta;
+  q.beta2 = 2.0f * p.beta;
+  q.h = p.h;
+  q.r1 = p.r1;
+  q.r2 = p.r2;
+  q.inv_span = 1.0f / (p.r2 - p.r1);
+  q.pif_half = 0.5f * p.pi_factor;
+  return q;
+}
+
+// fc and dfc/dr (src/minimal_tersoff.cu:77-90)
+__device__ __forceinline__ void tersoff_fc(const TLane& q, float d, float& fc, float& fcp)
+{
+  fc = 1.0f;
+  fcp = 0.0f;
+  if (d >= q.r1) {
+    if (d < q.r2) {
+      float sn, cs;
+      sincospi01((d - q.r1) * q.inv_span, sn, cs);
+      fc = 0.5f * cs + 0.5f;
+      fcp = -q.pif_half * sn;
+    } else {
+      fc = 0.0f;
+    }
   }
 }
 
-// b = (1 + zeta^n)^(-1/2n), b' = db/dzeta; b = 1, b' = 0 below zeta = 1e-16
-__device__ inline void tersoff_bond_order(const TersoffParams& q, float zeta, float& b, float& bp)
+// b = (1 + zeta^n)^(-1/2n) and db/dzeta; b = 1, b' = 0 below zeta = 1e-16
+// (src/minimal_tersoff.cu:174-185)
+__device__ __forceinline__ void tersoff_bond_order(const TLane& q, float zeta, float& b, float& bp)
 {
   if (zeta < 1.0e-16f) {
     b = 1.0f;
     bp = 0.0f;
   } else {
-    const float bzn = exp2f(q.en * log2f(zeta));
+    const float bzn = fexp2(q.en * flog2(zeta));
     const float one = 1.0f + bzn;
-    b = exp2f(q.minus_half_over_n * log2f(one));
-    bp = -b * bzn * 0.5f / (one * zeta);
+    b = fexp2(q.mhon * flog2(one));
+    bp = -0.5f * b * bzn * frcp(one * zeta);
   }
 }
 
-__device__ inline void tersoff_pair(const TersoffParams& q, int unlike, float d, float& fr,
-                                    float& frp, float& fa, float& fap)
+// fR, fR', fA, fA' (src/minimal_tersoff.cu:62-75) of parameter set k
+__device__ __forceinline__ void tersoff_pair(const TLane& q, int k, float d, float& fr, float& frp,
+                                             float& fa, float& fap)
 {
-  const float cR = unlike ? q.cR[1] : q.cR[0];
-  const float kR = unlike ? q.kR[1] : q.kR[0];
-  const float cA = unlike ? q.cA[1] : q.cA[0];
-  const float kA = unlike ? q.kA[1] : q.kA[0];
-  const float x = d - (unlike ? q.r0[1] : q.r0[0]);
-  fr = cR * expf(kR * x);
-  frp = kR * fr;
-  fa = cA * expf(kA * x);
-  fap = kA * fa;
+  const float x = d - q.r0[k];
+  fr = q.cR[k] * fexp2(q.kR2[k] * x);
+  frp = q.kR[k] * fr;
+  fa = q.cA[k] * fexp2(q.kA2[k] * x);
+  fap = q.kA[k] * fa;
 }
 
 // --------------------------------------- single-parameter-set kernels
 // Thread per atom; scratch per bond: b, b' then f_ij.
 
-__global__ void tersoff_plain_bond_order(DevSet s, TersoffParams q, float* __restrict__ gb,
+__global__ void tersoff_plain_bond_order(DevSet s, TLane q, float* __restrict__ gb,
                                          float* __restrict__ gbp)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -87,8 +119,7 @@ __global__ void tersoff_plain_bond_order(DevSet s, TersoffParams q, float* __res
   }
 }
 
-__global__ void tersoff_plain_partial_force(DevSet s, TersoffParams q,
-                                            const float* __restrict__ gb,
+__global__ void tersoff_plain_partial_force(DevSet s, TLane q, const float* __restrict__ gb,
                                             const float* __restrict__ gbp,
                                             float* __restrict__ f12, float* __restrict__ pe)
 {
@@ -99,7 +130,7 @@ __global__ void tersoff_plain_partial_force(DevSet s, TersoffParams q,
   float e = 0.0f;
   for (int u = 0; u < nn; ++u) {
     const float4 r = s.bond[b0 + u];
-    const float dinv = 1.0f / r.w;
+    const float dinv = frcp(r.w);
     float fc, fcp, fr, frp, fa, fap;
     tersoff_fc(q, r.w, fc, fcp);
     tersoff_pair(q, s.ptype[b0 + u], r.w, fr, frp, fa, fap);
@@ -115,7 +146,7 @@ __global__ void tersoff_plain_partial_force(DevSet s, TersoffParams q,
       const float av = gbp[b0 + v] * fav;
       const float4 t = T[u * nn + v];
       const float x = t.x - q.h;
-      const float g = q.beta * x * x, gp = 2.0f * q.beta * x;
+      const float g = q.beta * x * x, gp = q.beta2 * x;
       const float w = fcv * (au + av) * gp;
       cx += w * t.y;
       cy += w * t.z;
@@ -170,41 +201,47 @@ void launch_tersoff_plain(const DevSet& s, const TersoffParams& q, float* b, flo
                           float* f12, float* f, float* virial, float* pe, hipStream_t st)
 {
   const int block = 128, grid = (s.N + block - 1) / block;
-  tersoff_plain_bond_order<<<grid, block, 0, st>>>(s, q, b, bp);
-  tersoff_plain_partial_force<<<grid, block, 0, st>>>(s, q, b, bp, f12, pe);
+  const TLane l = tersoff_lane(q);
+  tersoff_plain_bond_order<<<grid, block, 0, st>>>(s, l, b, bp);
+  tersoff_plain_partial_force<<<grid, block, 0, st>>>(s, l, b, bp, f12, pe);
   tersoff_plain_assemble<<<grid, block, 0, st>>>(s, f12, f, virial);
   GPUGA_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------- population
 
+// NMAX bounds the neighbour count of the training set so the per-bond values
+// of an atom live in registers; TWO selects the two-species parameter sets.
+template <int NMAX, bool TWO>
 struct TersoffPop {
   static constexpr bool kForcePass = true;
-  using Lane = TersoffParams;
+  using Lane = TLane;
   using Extra = int;
 
   __device__ static Lane make(const PopCtl& ctl, int p)
   {
-    float x[13];
+    constexpr int V = TWO ? 13 : 9;
+    float x[V];
 #pragma unroll
-    for (int v = 0; v < 13; ++v)
-      x[v] = v < ctl.V ? denormalise(ctl.pmin[v], ctl.pmax[v], ctl.genes[(size_t)p * ctl.V + v])
-                       : 0.0f;
-    return tersoff_params(ctl.V == 13 ? 2 : 1, x);
+    for (int v = 0; v < V; ++v)
+      x[v] = denormalise(ldu(ctl.pmin, v), ldu(ctl.pmax, v), ctl.genes[(size_t)p * V + v]);
+    return tersoff_lane(tersoff_params(TWO ? 2 : 1, x));
   }
 
-  // One atom for 64 individuals.  NMAX bounds the neighbour count so that the
-  // per-bond values live in registers; all loop guards are wave-uniform.
-  template <int NMAX, bool FORCE>
-  __device__ static void atom(const DevSet& s, const TersoffParams& q, int i, float* fblk,
-                              int lane, Acc& acc)
+  // One atom for 64 individuals: the geometry is wave-uniform (scalar loads),
+  // the parameters are per lane; all loop guards are scalar branches.
+  template <bool FORCE>
+  __device__ static void atom(const DevSet& s, const TLane& q, int i, float* fblk, int lane,
+                              Acc& acc)
   {
-    const int nn = s.nn[i], b0 = s.bond0[i];
-    const float4* __restrict__ T = s.tri + s.tri0[i];
-    float fc[NMAX], fcp[NMAX], A[NMAX], B[NMAX], pf[NMAX];
+    const int nn = ldu(s.nn, i), b0 = ldu(s.bond0, i), t0 = ldu(s.tri0, i);
+    float fc[NMAX], fcp[NMAX], fcb[NMAX], A[NMAX], B[NMAX], pf[NMAX];
 #pragma unroll
     for (int u = 0; u < NMAX; ++u) {
-      if (u < nn) tersoff_fc(q, s.bond[b0 + u].w, fc[u], fcp[u]);
+      if (u < nn) {
+        tersoff_fc(q, ldu(s.bond, b0 + u).w, fc[u], fcp[u]);
+        fcb[u] = fc[u] * q.beta;
+      }
     }
     float e = 0.0f;
 #pragma unroll
@@ -214,18 +251,21 @@ struct TersoffPop {
 #pragma unroll
         for (int v = 0; v < NMAX; ++v) {
           if (v < nn && v != u) {
-            const float x = T[u * nn + v].x - q.h;
-            zeta += fc[v] * (q.beta * x * x);
+            const float x = ldu(s.tri, t0 + u * nn + v).x - q.h;
+            zeta = fmaf(fcb[v], x * x, zeta);
           }
         }
         float b, bp;
         tersoff_bond_order(q, zeta, b, bp);
-        const float d = s.bond[b0 + u].w;
+        const float d = ldu(s.bond, b0 + u).w;
+        const int k = TWO ? ldu(s.ptype, b0 + u) : 0;
         float fr, frp, fa, fap;
-        tersoff_pair(q, s.ptype[b0 + u], d, fr, frp, fa, fap);
+        tersoff_pair(q, k, d, fr, frp, fa, fap);
         const float rep = fr - b * fa;
-        e += fc[u] * rep;
-        pf[u] = (fcp[u] * rep + fc[u] * (frp - b * fap)) / d;
+        const float dinv = frcp(d);
+        e = fmaf(fc[u], rep, e);
+        pf[u] = (fcp[u] * rep + fc[u] * (frp - b * fap)) * dinv;
+        fcp[u] *= dinv;
         A[u] = bp * fa;
         B[u] = A[u] * fc[u];
       }
@@ -238,18 +278,17 @@ struct TersoffPop {
 #pragma unroll
         for (int v = 0; v < NMAX; ++v) {
           if (v < nn && v != u) {
-            const float4 t = T[u * nn + v];
+            const float4 t = ldu(s.tri, t0 + u * nn + v);
             const float x = t.x - q.h;
-            const float g = q.beta * x * x, gp = 2.0f * q.beta * x;
-            const float w = fc[v] * (A[u] + A[v]) * gp;
-            cx += w * t.y;
-            cy += w * t.z;
-            cz += w * t.w;
-            dsum += B[v] * g;
+            const float w = fc[v] * (A[u] + A[v]) * (q.beta2 * x);
+            cx = fmaf(w, t.y, cx);
+            cy = fmaf(w, t.z, cy);
+            cz = fmaf(w, t.w, cz);
+            dsum = fmaf(B[v], q.beta * x * x, dsum);
           }
         }
-        const float4 r = s.bond[b0 + u];
-        const float coef = 0.5f * (pf[u] - fcp[u] / r.w * dsum);
+        const float4 r = ldu(s.bond, b0 + u);
+        const float coef = 0.5f * (pf[u] - fcp[u] * dsum);
         const float hc = 0.5f * fc[u];
         const float fx = coef * r.x - hc * cx;
         const float fy = coef * r.y - hc * cy;
@@ -282,26 +321,15 @@ struct TersoffPop {
     }
   }
 
-  template <int NMAX>
-  __device__ static void run(const DevSet& s, const TersoffParams& q, const WorkItem& it,
-                             const PopCtl& ctl, Acc& acc)
+  __device__ static void chunk(const DevSet& s, const TLane& q, const WorkItem& it,
+                               const PopCtl& ctl, int, Acc& acc)
   {
     const int lane = __lane_id();
     if (it.c < s.Nc_force) {
       float* fblk = ctl.bondf + (size_t)it.k * ctl.B_force * 3 * WAVE;
-      for (int i = it.a0; i < it.a1; ++i) atom<NMAX, true>(s, q, i, fblk, lane, acc);
+      for (int i = it.a0; i < it.a1; ++i) atom<true>(s, q, i, fblk, lane, acc);
     } else {
-      for (int i = it.a0; i < it.a1; ++i) atom<NMAX, false>(s, q, i, nullptr, lane, acc);
-    }
-  }
-
-  __device__ static void chunk(const DevSet& s, const TersoffParams& q, const WorkItem& it,
-                               const PopCtl& ctl, int, Acc& acc)
-  {
-    switch (it.cls) {
-    case 0: run<4>(s, q, it, ctl, acc); break;
-    case 1: run<8>(s, q, it, ctl, acc); break;
-    default: run<16>(s, q, it, ctl, acc); break;
+      for (int i = it.a0; i < it.a1; ++i) atom<false>(s, q, i, nullptr, lane, acc);
     }
   }
 
@@ -311,42 +339,69 @@ struct TersoffPop {
   __device__ static double force_pass(const DevSet& s, const PopCtl& ctl, int c, int k, int lane)
   {
     const float* fblk = ctl.bondf + (size_t)k * ctl.B_force * 3 * WAVE + lane;
-    const int a0 = s.Na_sum[c], a1 = a0 + s.Na[c];
-    const size_t N = s.N;
+    const int a0 = ldu(s.Na_sum, c), a1 = a0 + ldu(s.Na, c);
+    const long N = s.N;
     double err = 0.0;
     for (int i = a0; i < a1; ++i) {
-      const int nn = s.nn[i], b0 = s.bond0[i];
+      const int nn = ldu(s.nn, i), b0 = ldu(s.bond0, i);
       float fx = 0.0f, fy = 0.0f, fz = 0.0f;
       for (int u = 0; u < nn; ++u) {
         const float* a = fblk + (size_t)(b0 + u) * 3 * WAVE;
-        const float* r = fblk + (size_t)s.rev[b0 + u] * 3 * WAVE;
+        const float* r = fblk + (size_t)ldu(s.rev, b0 + u) * 3 * WAVE;
         fx += a[0] - r[0];
         fy += a[WAVE] - r[WAVE];
         fz += a[2 * WAVE] - r[2 * WAVE];
       }
-      const float dx = fx - s.force_ref[i];
-      const float dy = fy - s.force_ref[i + N];
-      const float dz = fz - s.force_ref[i + 2 * N];
+      const float dx = fx - ldu(s.force_ref, i);
+      const float dy = fy - ldu(s.force_ref, i + N);
+      const float dz = fz - ldu(s.force_ref, i + 2 * N);
       err += (double)dx * dx + (double)dy * dy + (double)dz * dz;
     }
     return err;
   }
 };
 
-void launch_tersoff_population(const PopCtl& ctl, const DevSet& s, int grid, int block,
+template <int NMAX, bool TWO>
+static void launch_t(const PopCtl& ctl, const DevSet& s, int grid, int block, hipStream_t st)
+{
+  population_kernel<TersoffPop<NMAX, TWO>><<<grid, block, 0, st>>>(ctl, s, 0);
+}
+
+// cls: 0 for at most 4 neighbours per atom, 1 for 8, 2 for 16
+void launch_tersoff_population(const PopCtl& ctl, const DevSet& s, int cls, int grid, int block,
                                hipStream_t st)
 {
-  population_kernel<TersoffPop><<<grid, block, 0, st>>>(ctl, s, 0);
+  const bool two = ctl.V == 13;
+  switch (cls * 2 + (two ? 1 : 0)) {
+  case 0: launch_t<4, false>(ctl, s, grid, block, st); break;
+  case 1: launch_t<4, true>(ctl, s, grid, block, st); break;
+  case 2: launch_t<8, false>(ctl, s, grid, block, st); break;
+  case 3: launch_t<8, true>(ctl, s, grid, block, st); break;
+  case 4: launch_t<16, false>(ctl, s, grid, block, st); break;
+  default: launch_t<16, true>(ctl, s, grid, block, st); break;
+  }
   GPUGA_HIP(hipGetLastError());
 }
 
-} // namespace gpuga
-
-namespace gpuga {
-int tersoff_population_occupancy(int block)
+template <int NMAX, bool TWO>
+static int occ_t(int block)
 {
   int n = 0;
-  GPUGA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, population_kernel<TersoffPop>, block, 0));
+  GPUGA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+    &n, reinterpret_cast<const void*>(&population_kernel<TersoffPop<NMAX, TWO>>), block, 0));
   return n;
 }
+
+int tersoff_population_occupancy(int cls, bool two, int block)
+{
+  switch (cls * 2 + (two ? 1 : 0)) {
+  case 0: return occ_t<4, false>(block);
+  case 1: return occ_t<4, true>(block);
+  case 2: return occ_t<8, false>(block);
+  case 3: return occ_t<8, true>(block);
+  case 4: return occ_t<16, false>(block);
+  default: return occ_t<16, true>(block);
+  }
+}
+
 } // namespace gpuga
