@@ -85,6 +85,7 @@ def lib():
             "gpuga_ga_set_population": (C.c_int, [vp, fp]),
             "gpuga_ga_step": (C.c_int, [vp, C.c_int, fp, fp]),
             "gpuga_run": (C.c_int, [C.c_char_p, P(Options), dp]),
+            "gpuga_read_ga_in": (C.c_int, [C.c_char_p, ip, ip, ip, fp, P(C.c_uint)]),
             "gpuga_server_create": (C.c_int, [C.c_double, C.c_int, P(Options), P(vp)]),
             "gpuga_server_destroy": (None, [vp]),
             "gpuga_server_submit": (C.c_int, [vp, C.c_char_p, C.c_int, ip]),
@@ -286,6 +287,16 @@ def inspect_neighbors(input_dir):
     _check(lib().gpuga_inspect_neighbors(d, _ptr(sizes, C.c_int), _ptr(NN, C.c_int),
                                          _ptr(start, C.c_int), _ptr(NL, C.c_int)))
     return NN, start, NL[:B]
+
+
+def read_ga_in(input_dir):
+    """ga.in of a job directory (host only)."""
+    G, P, par, seed = C.c_int(), C.c_int(), C.c_int(), C.c_uint()
+    rate = C.c_float()
+    _check(lib().gpuga_read_ga_in(str(input_dir).encode(), C.byref(G), C.byref(P), C.byref(par),
+                                  C.byref(rate), C.byref(seed)))
+    return {"maximum_generation": G.value, "population_size": P.value,
+            "parent_number": par.value, "mutation_rate": rate.value, "seed": seed.value}
 
 
 def run(input_dir, **opts):

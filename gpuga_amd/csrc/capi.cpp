@@ -360,6 +360,19 @@ int gpuga_ga_step(gpuga_ga* g, int generation, const float* fitness, float* sort
   });
 }
 
+int gpuga_read_ga_in(const char* dir, int* G, int* P, int* parents, float* rate, unsigned* seed)
+{
+  return guarded([&] {
+    need(dir, "input_dir");
+    const GAParams p = read_ga(dir);
+    if (G) *G = p.maximum_generation;
+    if (P) *P = p.population_size;
+    if (parents) *parents = p.parent_number;
+    if (rate) *rate = p.mutation_rate;
+    if (seed) *seed = p.seed;
+  });
+}
+
 int gpuga_run(const char* input_dir, const gpuga_options* opt, double* seconds_per_generation)
 {
   return guarded([&] {

@@ -26,6 +26,13 @@ struct DevSet {
   const float4* bond;     // [B] r_ij.x, r_ij.y, r_ij.z, |r_ij|
   const int* tri0;        // [N] first triplet of atom i (Tersoff only)
   const float4* tri;      // [T] cos theta, d cos / d r_ij (Tersoff only)
+  // Tersoff atom records, one per atom at a fixed stride of 1 + M + M*M
+  // float4 (M = neighbour bound of the set): header {nn, unlike-type bit
+  // mask, first bond, 0}, then M bonds (x, y, z, d), then the M x M
+  // triplets (cos theta_jik, d cos / d r_ij).  Every load of an atom is
+  // independent of the others, so they issue together.
+  const float4* rec;
+  int rec_nmax;
   const float* pe_ref;    // [Nc]
   const float* virial_ref;// [6][Nc]
   const float* force_ref; // [3][N]

@@ -119,7 +119,7 @@ size_t DeviceSet::bytes() const
 {
   return Na.bytes() + Na_sum.bytes() + type.bytes() + nn.bytes() + bond0.bytes() + nl.bytes() +
          rev.bytes() + ptype.bytes() + tri0.bytes() + bond.bytes() + tri.bytes() +
-         ri_tab.bytes() + pe_ref.bytes() + virial_ref.bytes() + force_ref.bytes();
+         ri_tab.bytes() + rec.bytes() + pe_ref.bytes() + virial_ref.bytes() + force_ref.bytes();
 }
 
 PopPlan::~PopPlan()
@@ -205,6 +205,26 @@ void Fitness::setup()
     ds_.tri0.upload(geo_.tri_start, st);
     if (geo_.tri.empty()) geo_.tri.push_back(Vec4{0, 0, 0, 0});
     ds_.tri.upload((const float4*)geo_.tri.data(), geo_.tri.size(), st);
+    if (nl_.max_nn <= 16) {
+      // fixed-stride atom records for the population kernel (see DevSet::rec)
+      const int M = nl_.max_nn <= 4 ? 4 : (nl_.max_nn <= 8 ? 8 : 16);
+      const size_t RS = 1 + M + (size_t)M * M;
+      std::vector<Vec4> rec((size_t)ts_.N * RS, Vec4{0, 0, 0, 0});
+      for (int i = 0; i < ts_.N; ++i) {
+        const int nn = nl_.NN[i], b0 = nl_.start[i];
+        Vec4* R = &rec[(size_t)i * RS];
+        int hdr[4] = {nn, 0, b0, 0};
+        for (int u = 0; u < nn; ++u) {
+          hdr[1] |= (ptype[b0 + u] & 1) << u;
+          R[1 + u] = geo_.bond[b0 + u];
+          for (int w = 0; w < nn; ++w)
+            if (w != u) R[1 + M + u * M + w] = geo_.tri[geo_.tri_start[i] + u * nn + w];
+        }
+        std::memcpy(&R[0], hdr, sizeof hdr);
+      }
+      ds_.rec.upload((const float4*)rec.data(), rec.size(), st);
+      ds_.view.rec_nmax = M;
+    }
   } else {
     ds_.tri0.upload(std::vector<int>(ts_.N, 0), st);
     ds_.tri.upload(std::vector<float4>(1), st);
@@ -242,6 +262,7 @@ void Fitness::setup()
   v.bond = ds_.bond.get();
   v.tri0 = ds_.tri0.get();
   v.tri = ds_.tri.get();
+  v.rec = ds_.rec.get();
   v.pe_ref = ds_.pe_ref.get();
   v.virial_ref = ds_.virial_ref.get();
   v.force_ref = ds_.force_ref.get();

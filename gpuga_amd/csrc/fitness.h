@@ -26,7 +26,7 @@ struct FitnessOptions {
 // Device-resident copy of one training set and its geometry tables.
 struct DeviceSet {
   DeviceBuffer<int> Na, Na_sum, type, nn, bond0, nl, rev, ptype, tri0;
-  DeviceBuffer<float4> bond, tri, ri_tab;
+  DeviceBuffer<float4> bond, tri, ri_tab, rec;
   DeviceBuffer<float> pe_ref, virial_ref, force_ref;
   DevSet view{};
   size_t bytes() const;

@@ -228,18 +228,23 @@ struct TersoffPop {
     return tersoff_lane(tersoff_params(TWO ? 2 : 1, x));
   }
 
-  // One atom for 64 individuals: the geometry is wave-uniform (scalar loads),
-  // the parameters are per lane; all loop guards are scalar branches.
+  // One atom for 64 individuals: the geometry is wave-uniform (scalar loads
+  // from the atom's record), the parameters are per lane; all loop guards are
+  // scalar branches.
+  static constexpr int RS = 1 + NMAX + NMAX * NMAX; // record stride in float4
+
   template <bool FORCE>
   __device__ static void atom(const DevSet& s, const TLane& q, int i, float* fblk, int lane,
                               Acc& acc)
   {
-    const int nn = ldu(s.nn, i), b0 = ldu(s.bond0, i), t0 = ldu(s.tri0, i);
+    const float4* R = s.rec + (long)i * RS;
+    const int4 hd = ldu(reinterpret_cast<const int4*>(R), 0);
+    const int nn = hd.x, unl = hd.y, b0 = hd.z;
     float fc[NMAX], fcp[NMAX], fcb[NMAX], A[NMAX], B[NMAX], pf[NMAX];
 #pragma unroll
     for (int u = 0; u < NMAX; ++u) {
       if (u < nn) {
-        tersoff_fc(q, ldu(s.bond, b0 + u).w, fc[u], fcp[u]);
+        tersoff_fc(q, ldu(R, 1 + u).w, fc[u], fcp[u]);
         fcb[u] = fc[u] * q.beta;
       }
     }
@@ -251,14 +256,14 @@ struct TersoffPop {
 #pragma unroll
         for (int v = 0; v < NMAX; ++v) {
           if (v < nn && v != u) {
-            const float x = ldu(s.tri, t0 + u * nn + v).x - q.h;
+            const float x = ldu(R, 1 + NMAX + u * NMAX + v).x - q.h;
             zeta = fmaf(fcb[v], x * x, zeta);
           }
         }
         float b, bp;
         tersoff_bond_order(q, zeta, b, bp);
-        const float d = ldu(s.bond, b0 + u).w;
-        const int k = TWO ? ldu(s.ptype, b0 + u) : 0;
+        const float d = ldu(R, 1 + u).w;
+        const int k = TWO ? ((unl >> u) & 1) : 0;
         float fr, frp, fa, fap;
         tersoff_pair(q, k, d, fr, frp, fa, fap);
         const float rep = fr - b * fa;
@@ -278,7 +283,7 @@ struct TersoffPop {
 #pragma unroll
         for (int v = 0; v < NMAX; ++v) {
           if (v < nn && v != u) {
-            const float4 t = ldu(s.tri, t0 + u * nn + v);
+            const float4 t = ldu(R, 1 + NMAX + u * NMAX + v);
             const float x = t.x - q.h;
             const float w = fc[v] * (A[u] + A[v]) * (q.beta2 * x);
             cx = fmaf(w, t.y, cx);
@@ -287,7 +292,7 @@ struct TersoffPop {
             dsum = fmaf(B[v], q.beta * x * x, dsum);
           }
         }
-        const float4 r = ldu(s.bond, b0 + u);
+        const float4 r = ldu(R, 1 + u);
         const float coef = 0.5f * (pf[u] - fcp[u] * dsum);
         const float hc = 0.5f * fc[u];
         const float fx = coef * r.x - hc * cx;

@@ -119,6 +119,12 @@ int gpuga_ga_set_population(gpuga_ga* g, const float* genes);
  * mutation(generation) (src/ga.cu:83-88).  sorted_fitness[P] may be NULL. */
 int gpuga_ga_step(gpuga_ga* g, int generation, const float* fitness, float* sorted_fitness);
 
+/* ga.in reader of GA::read_parameters (src/ga.cu:214-316); host only.  Also
+ * accepts an optional `seed` keyword (default 12345678, the reference's
+ * deterministic seed, src/ga.cu:47-48). */
+int gpuga_read_ga_in(const char* input_dir, int* maximum_generation, int* population_size,
+                     int* parent_number, float* mutation_rate, unsigned* seed);
+
 /* GPUGA::GPUGA(char* dir), src/gpuga.cu:24-28: reads ga.in, potential.in,
  * train.in from dir, runs the GA, writes ga.out, force.out, energy.out,
  * virial.out.  seconds_per_generation may be NULL. */

@@ -1,0 +1,128 @@
+"""Host side of the library, no GPU needed: the C ABI exports what
+include/gpuga.h declares, the set-up path (readers, neighbour list) agrees
+bit for bit with the oracle, and malformed inputs fail with the reference's
+messages (src/fitness.cu:163-314, src/ga.cu:214-316)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import gpuga_amd as G
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "gpuga.h")).read()
+    return sorted(set(re.findall(r"\b(gpuga_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_symbols()
+    assert len(names) >= 30
+    out = subprocess.run(["nm", "-D", "--defined-only", G.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (gpuga_[a-z_0-9]+)\b", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    lib = G.lib()
+    for n in names:
+        assert hasattr(lib, n)
+
+
+@pytest.mark.parametrize("name", ["si_diamond", "gaas", "si_type2", "nn2b", "hfo2"])
+def test_neighbor_list_bit_exact_with_oracle(golden, oracle_case, name):
+    NN, start, NL = G.inspect_neighbors(golden(name))
+    a, b, c = oracle_case(name).neighbors()
+    assert np.array_equal(NN, a) and np.array_equal(start, b) and np.array_equal(NL, c)
+
+
+def test_synthetic_sets_round_trip_through_both_readers(tmp_path):
+    from gpuga_amd import synth
+    d = synth.tersoff_si(Nc=12, sizes="loguniform", seed=4)
+    d = synth.label(d, O.Case(potential=dict(potential_type=1, cutoff=3.0,
+                                             weights=(0.05, 0.15, 0.8), pmin=d["pmin"],
+                                             pmax=d["pmax"]), arrays=d).find_force)
+    synth.write_dir(d, str(tmp_path), ga={"maximum_generation": 5, "population_size": 20,
+                                          "parent_number": 10, "mutation_rate": 0.2})
+    case = O.Case(str(tmp_path))
+    assert case.N == int(np.sum(d["Na"]))
+    # float values survive the text round trip bit for bit
+    assert np.array_equal(case.array("r", 3 * case.N), np.asarray(d["r"], np.float32))
+    ncf = int(d["Nc_force"])  # force configurations carry no energy line
+    assert np.array_equal(case.array("pe_ref", case.Nc)[ncf:], d["pe_ref"][ncf:])
+    NN, start, NL = G.inspect_neighbors(str(tmp_path))
+    a, b, c = case.neighbors()
+    assert np.array_equal(NN, a) and np.array_equal(NL, c)
+    ga = G.read_ga_in(str(tmp_path))
+    assert ga["population_size"] == 20 and ga["seed"] == 12345678
+
+
+def write(path, name, text):
+    os.makedirs(path, exist_ok=True)
+    with open(os.path.join(path, name), "w") as f:
+        f.write(text)
+
+
+POT = "potential_type 1\ncutoff 3.0\nweight_force 0.05\nweight_energy 0.15\nweight_virial 0.8\n" + \
+      "".join(f"p{k} 1 2\n" for k in range(9))
+
+
+@pytest.mark.parametrize("train,msg", [
+    ("1 1\n2\n", "Number of configurations should >= 2"),
+    ("2 0\n2\n2\n", "Number of force configurations should >= 1"),
+    ("2 2\n2\n2\n", "Number of potential configurations should >= 1"),
+    ("2 1\n1\n2\n", "Number of atoms 1 should >= 2"),
+    ("2 1\n2\n2\n10 0 0 0 10 0 0 0 10\n0 0 0 0 0 0 0\n", "reading error for train.in"),
+])
+def test_train_in_errors(tmp_path, train, msg):
+    d = str(tmp_path)
+    write(d, "potential.in", POT)
+    write(d, "train.in", train)
+    with pytest.raises(G.GpugaError, match=msg):
+        G.inspect_neighbors(d)
+
+
+@pytest.mark.parametrize("pot,msg", [
+    ("potential_type 7\n", "unsupported potential type"),
+    ("potential_type 1\ncutoff 3\nweight_force -1\nweight_energy 0\nweight_virial 0\n",
+     "weight for force should >= 0"),
+    ("potential_type 1\ncutoff 3\nweight_force 1\nweight_energy 1\nweight_virial 1\np 1\n",
+     "reading error for potential.in"),
+])
+def test_potential_in_errors(tmp_path, pot, msg):
+    d = str(tmp_path)
+    write(d, "potential.in", pot)
+    write(d, "train.in", "2 1\n2\n2\n")
+    with pytest.raises(G.GpugaError, match=msg):
+        G.inspect_neighbors(d)
+
+
+@pytest.mark.parametrize("ga,msg", [
+    ("generations 10\n", "'generations' is invalid keyword"),
+    ("population_size 10\n", "population_size should >= 20"),
+    ("population_size 30\nparent_number 11\n", "population_size - parent_number should be even"),
+    ("mutation_rate 2\n", "mutation_rate should <= 1"),
+    ("maximum_generation 0\n", "maximum_generation should be positive"),
+    ("maximum_generation ten\n", "maximum_generation should be integer"),
+    ("population_size 20 30\n", "population_size should have 1 parameter"),
+])
+def test_ga_in_errors(tmp_path, ga, msg):
+    d = str(tmp_path)
+    write(d, "ga.in", ga)
+    with pytest.raises(G.GpugaError, match=msg):
+        G.read_ga_in(d)
+
+
+def test_ga_in_comments_defaults_and_seed(tmp_path):
+    d = str(tmp_path)
+    write(d, "ga.in", "# comment line\n\nmaximum_generation 7  # trailing\npopulation_size 256\n"
+                      "parent_number 128\nseed 99\n")
+    ga = G.read_ga_in(d)
+    assert ga == {"maximum_generation": 7, "population_size": 256, "parent_number": 128,
+                  "mutation_rate": pytest.approx(0.2), "seed": 99}
+    og = O.lib()
+    del og
