@@ -208,7 +208,7 @@ void Fitness::setup()
     if (nl_.max_nn <= 16) {
       // fixed-stride atom records for the population kernel (see DevSet::rec)
       const int M = nl_.max_nn <= 4 ? 4 : (nl_.max_nn <= 8 ? 8 : 16);
-      const size_t RS = 1 + M + (size_t)M * M;
+      const size_t RS = 1 + M + (size_t)M * M + M / 4;
       std::vector<Vec4> rec((size_t)ts_.N * RS, Vec4{0, 0, 0, 0});
       for (int i = 0; i < ts_.N; ++i) {
         const int nn = nl_.NN[i], b0 = nl_.start[i];
@@ -221,6 +221,8 @@ void Fitness::setup()
             if (w != u) R[1 + M + u * M + w] = geo_.tri[geo_.tri_start[i] + u * nn + w];
         }
         std::memcpy(&R[0], hdr, sizeof hdr);
+        int* rv = reinterpret_cast<int*>(&R[1 + M + M * M]);
+        for (int u = 0; u < nn; ++u) rv[u] = nl_.rev[b0 + u];
       }
       ds_.rec.upload((const float4*)rec.data(), rec.size(), st);
       ds_.view.rec_nmax = M;
@@ -401,10 +403,17 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   std::stable_sort(chunks.begin(), chunks.end(),
                    [](const Chunk& a, const Chunk& b) { return a.cost > b.cost; });
   std::vector<WorkItem> items;
-  items.reserve(chunks.size() * K);
+  items.reserve(chunks.size() * K * 2);
   for (const Chunk& ch : chunks)
     for (int k = 0; k < K; ++k)
       items.push_back(WorkItem{ch.c, k, ch.chunk, ch.a0, ch.a1, ch.cls, 0, 0});
+  // three-body force configurations: force-assembly items after every
+  // evaluation item (the order the kernel's dependency waits rely on)
+  if (tersoff)
+    for (const Chunk& ch : chunks)
+      if (ch.c < ts_.Nc_force)
+        for (int k = 0; k < K; ++k)
+          items.push_back(WorkItem{ch.c, k, ch.chunk, ch.a0, ch.a1, ch.cls, 1, 0});
   pl.n_items = (int)items.size();
 
   hipStream_t st = stream_;
@@ -420,11 +429,17 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
     ts_.N_force > 0 ? nl_.start[ts_.N_force - 1] + nl_.NN[ts_.N_force - 1] : 0;
   pl.bondf.resize(tersoff ? std::max<size_t>((size_t)K * B_force * 3 * WAVE, 1) : 1);
   if (spec_.type == POT_NN2B) pl.nn2b_table.resize((size_t)K * nn2b_grid_desc().G * 3 * WAVE);
-  pl.counters.resize((size_t)Nc * K + K + 1);
+  const int groups = (Nc + CFG_GROUP - 1) / CFG_GROUP;
+  pl.gpart.resize((size_t)K * groups * 3 * WAVE);
+  // counters: cnt [Nc*K] | cnt2 [Nc*K] | grp_done [K*groups] | blk_done [K] |
+  // fetch | status
+  pl.counters.resize((size_t)2 * Nc * K + (size_t)K * groups + K + 2);
   pl.counters.zero(st);
   pl.h_genes.resize((size_t)P * spec_.V);
   pl.h_fitness.resize(P);
   pl.h_terms.resize(3 * (size_t)P);
+  pl.h_status.resize(1);
+  pl.h_status[0] = 0u;
 
   PopCtl& c = pl.ctl;
   c.items = pl.items.get();
@@ -441,9 +456,14 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   c.bondf = pl.bondf.get();
   c.B_force = B_force;
   c.err = pl.err.get();
-  c.cnt = pl.counters.get();
-  c.blk_done = pl.counters.get() + (size_t)Nc * K;
-  c.fetch = pl.counters.get() + (size_t)Nc * K + K;
+  unsigned* ctr = pl.counters.get();
+  c.cnt = ctr;
+  c.cnt2 = ctr + (size_t)Nc * K;
+  c.grp_done = ctr + (size_t)2 * Nc * K;
+  c.blk_done = c.grp_done + (size_t)K * groups;
+  c.fetch = c.blk_done + K;
+  c.status = c.fetch + 1;
+  c.gpart = pl.gpart.get();
   c.fitness = pl.fitness.get();
   c.terms = pl.terms.get();
   c.norm = norm();
@@ -524,6 +544,8 @@ void Fitness::compute_async(int P, const float* genes)
                            hipMemcpyDeviceToHost, st));
   GPUGA_HIP(hipMemcpyAsync(pl.h_terms.get(), pl.terms.get(), sizeof(double) * 3 * P,
                            hipMemcpyDeviceToHost, st));
+  GPUGA_HIP(hipMemcpyAsync(pl.h_status.get(), pl.ctl.status, sizeof(unsigned),
+                           hipMemcpyDeviceToHost, st));
   pending_P_ = P;
 }
 
@@ -539,9 +561,16 @@ void Fitness::compute_wait(float* fitness, double* terms)
     kernel_ms_ += ms;
     ++kernel_launches_;
   }
+  pending_P_ = 0;
+  if (pl.h_status[0] != 0u) {
+    // a dependency wait of the population kernel ran out: the results of
+    // this launch are not valid; clear the flag and report
+    GPUGA_HIP(hipMemsetAsync(pl.ctl.status, 0, sizeof(unsigned), stream_));
+    GPUGA_HIP(hipStreamSynchronize(stream_));
+    fail("population kernel: force-assembly wait timed out; results discarded");
+  }
   if (fitness) std::memcpy(fitness, pl.h_fitness.get(), sizeof(float) * P);
   if (terms) std::memcpy(terms, pl.h_terms.get(), sizeof(double) * 3 * P);
-  pending_P_ = 0;
 }
 
 void Fitness::compute(int P, const float* genes, float* fitness, double* terms)

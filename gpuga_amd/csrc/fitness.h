@@ -51,11 +51,12 @@ struct PopPlan {
   int P = 0, K = 0, n_items = 0, grid = 0, block = 256, chunk_cost = 0;
   DeviceBuffer<WorkItem> items;
   DeviceBuffer<int> cfg_nchunks, cfg_slot0;
-  DeviceBuffer<double> part, err, terms;
+  DeviceBuffer<double> part, err, gpart, terms;
   DeviceBuffer<float> bondf, genes, fitness, nn2b_table;
   DeviceBuffer<unsigned> counters; // [Nc*K] cnt | [K] blk_done | [1] fetch
   PinnedBuffer<float> h_genes, h_fitness;
   PinnedBuffer<double> h_terms;
+  PinnedBuffer<unsigned> h_status;
   PopCtl ctl{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;

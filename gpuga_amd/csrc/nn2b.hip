@@ -187,7 +187,10 @@ struct NN2BPop {
     }
   }
 
-  __device__ static double force_pass(const DevSet&, const PopCtl&, int, int, int) { return 0.0; }
+  __device__ static double force_pass(const DevSet&, const PopCtl&, int, int, int, int)
+  {
+    return 0.0;
+  }
 };
 
 void launch_nn2b_tabulate_f32(const PopCtl& ctl, const NN2BGrid& g, float* table, hipStream_t st)

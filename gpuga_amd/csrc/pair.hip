@@ -146,7 +146,10 @@ struct RIPop {
     }
   }
 
-  __device__ static double force_pass(const DevSet&, const PopCtl&, int, int, int) { return 0.0; }
+  __device__ static double force_pass(const DevSet&, const PopCtl&, int, int, int, int)
+  {
+    return 0.0;
+  }
 };
 
 void launch_ri_plain(const DevSet& s, const RIParams& q, const float4* tab, float* f,

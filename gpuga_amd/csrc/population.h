@@ -9,14 +9,23 @@
 // differ per lane.  Per-configuration energy and virial are accumulated per
 // lane in FP64, so no cross-lane reduction is ever needed.
 //
-// Schedule.  Work items (configuration chunk x lane block) are sorted by cost,
-// largest first, and pulled from a global counter by the resident waves.  The
-// last chunk of a (configuration, lane block) to finish folds the chunk
-// partials in chunk order and writes that configuration's error terms; the
-// last configuration of a lane block to finish sums the error terms over the
-// configurations in index order and writes the fitness.  Every reduction has a
-// fixed order, so results do not depend on scheduling or on what else runs on
-// the device.  All counters are returned to zero by their last user, so the
+// Schedule.  Work items are pulled from a global counter by the resident
+// waves.  Phase-0 items (configuration chunk x lane block, sorted by cost,
+// largest first) evaluate the potential.  For three-body potentials the force
+// on an atom needs the partial forces of its neighbours, so phase-0 items of
+// force configurations only publish per-bond forces, and phase-1 items (queued
+// after every phase-0 item) assemble F_i - F_ref,i for their chunk once all
+// chunks of their configuration have published.  Because every phase-0 item
+// is handed out before any phase-1 item and phase-0 work never waits, each
+// such wait ends; it is bounded anyway, and a timeout is reported through a
+// status word instead of hanging.
+//
+// Reductions.  The last chunk of a (configuration, lane block) folds the chunk
+// partials in chunk order and writes the configuration's error terms; the last
+// configuration of a group of 64 sums the group in index order; the last group
+// of a lane block sums the groups and writes the fitness.  Every reduction has
+// a fixed order, so results do not depend on scheduling or on what else runs
+// on the device.  All counters are returned to zero by their last user, so the
 // launch can be replayed from a captured graph without clearing memory.
 //
 // Replaces the per-individual host loop of Fitness::compute
@@ -28,13 +37,16 @@
 
 namespace gpuga {
 
+constexpr int CFG_GROUP = 64; // configurations per reduction group
+
 struct WorkItem {
   int c;      // configuration
   int k;      // lane block
   int chunk;  // chunk index inside the configuration
   int a0, a1; // global atom range of the chunk
   int cls;    // neighbour-count class of the chunk
-  int pad0, pad1;
+  int phase;  // 0: evaluate, 1: assemble forces of a three-body force configuration
+  int pad;
 };
 
 struct PopCtl {
@@ -51,9 +63,13 @@ struct PopCtl {
   float* bondf;           // [K][B_force][3][64] per-bond forces (three-body only)
   int B_force;
   double* err;            // [K][3][Nc][64]
-  unsigned* cnt;          // [Nc][K]
+  double* gpart;          // [K][groups][3][64]
+  unsigned* cnt;          // [Nc][K]  phase-0 chunks done
+  unsigned* cnt2;         // [Nc][K]  phase-1 chunks done
+  unsigned* grp_done;     // [K][groups]
   unsigned* blk_done;     // [K]
   unsigned* fetch;        // [1]
+  unsigned* status;       // [1] non-zero: a dependency wait timed out
   float* fitness;         // [P]
   double* terms;          // [P][3]
   FitnessNorm norm;
@@ -71,58 +87,126 @@ __device__ inline unsigned wave_atomic_add(unsigned* p, unsigned x)
   return (unsigned)wave_first((int)old);
 }
 
-// Lane block k, all configurations finished: sum the error terms in
-// configuration order and write fitness and the three relative RMS errors.
-__device__ inline void finish_block(const PopCtl& ctl, int Nc, int k, int lane)
+// Wait (bounded, about a second) until *p >= target; false on timeout.
+__device__ inline bool wave_wait_count(const unsigned* p, unsigned target)
 {
-  const double* e0 = ctl.err + ((size_t)(k * 3 + 0) * Nc) * WAVE + lane;
-  const double* e1 = ctl.err + ((size_t)(k * 3 + 1) * Nc) * WAVE + lane;
-  const double* e2 = ctl.err + ((size_t)(k * 3 + 2) * Nc) * WAVE + lane;
+  for (int spin = 0; spin < (1 << 23); ++spin) {
+    unsigned v = 0;
+    if (__lane_id() == 0) v = __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    v = (unsigned)wave_first((int)v);
+    if (v >= target) return true;
+    __builtin_amdgcn_s_sleep(8);
+  }
+  return false;
+}
+
+// Sum n rows (stride `stride` doubles) of three arrays, 4 independent partial
+// sums per array so the loads overlap; fixed order.
+__device__ inline void sum3(const double* e0, const double* e1, const double* e2, long stride,
+                            int n, double out[3])
+{
   double s0[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
   int c = 0;
-  for (; c + 4 <= Nc; c += 4) {
+  for (; c + 4 <= n; c += 4) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      s0[u] += e0[(size_t)(c + u) * WAVE];
-      s1[u] += e1[(size_t)(c + u) * WAVE];
-      s2[u] += e2[(size_t)(c + u) * WAVE];
+      s0[u] += e0[(c + u) * stride];
+      s1[u] += e1[(c + u) * stride];
+      s2[u] += e2[(c + u) * stride];
     }
   }
-  for (; c < Nc; ++c) {
-    s0[0] += e0[(size_t)c * WAVE];
-    s1[0] += e1[(size_t)c * WAVE];
-    s2[0] += e2[(size_t)c * WAVE];
+  for (; c < n; ++c) {
+    s0[0] += e0[c * stride];
+    s1[0] += e1[c * stride];
+    s2[0] += e2[c * stride];
   }
-  const double se = (s0[0] + s0[1]) + (s0[2] + s0[3]);
-  const double sv = (s1[0] + s1[1]) + (s1[2] + s1[3]);
-  const double sf = (s2[0] + s2[1]) + (s2[2] + s2[3]);
-  const double tE = sqrt(se * ctl.norm.inv_pss);
-  const double tW = sqrt(sv * ctl.norm.inv_vss);
-  const double tF = sqrt(sf * ctl.norm.inv_fss);
+  out[0] = (s0[0] + s0[1]) + (s0[2] + s0[3]);
+  out[1] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
+  out[2] = (s2[0] + s2[1]) + (s2[2] + s2[3]);
+}
+
+// Configuration c of lane block k has its three error terms: propagate
+// through the group and lane-block reductions, writing the fitness when the
+// lane block is complete.
+__device__ inline void config_done(const PopCtl& ctl, int Nc, int c, int k, int lane, double eE,
+                                   double eW, double eF)
+{
+  const long E3 = (long)Nc * WAVE;
+  double* e = ctl.err + (long)k * 3 * E3 + (long)c * WAVE + lane;
+  e[0] = eE;
+  e[E3] = eW;
+  e[2 * E3] = eF;
+  const int groups = (Nc + CFG_GROUP - 1) / CFG_GROUP;
+  const int g = c / CFG_GROUP;
+  const int g0 = g * CFG_GROUP, gn = min(CFG_GROUP, Nc - g0);
+  __threadfence();
+  const unsigned gd = wave_atomic_add(&ctl.grp_done[k * groups + g], 1u);
+  if (gd != (unsigned)gn - 1u) return;
+  __threadfence();
+  if (lane == 0) ctl.grp_done[k * groups + g] = 0u;
+  const double* b = ctl.err + (long)k * 3 * E3 + (long)g0 * WAVE + lane;
+  double s[3];
+  sum3(b, b + E3, b + 2 * E3, WAVE, gn, s);
+  double* gp = ctl.gpart + ((long)k * groups + g) * 3 * WAVE + lane;
+  gp[0] = s[0];
+  gp[WAVE] = s[1];
+  gp[2 * WAVE] = s[2];
+  __threadfence();
+  const unsigned bd = wave_atomic_add(&ctl.blk_done[k], 1u);
+  if (bd != (unsigned)groups - 1u) return;
+  __threadfence();
+  if (lane == 0) ctl.blk_done[k] = 0u;
+  const double* gb = ctl.gpart + (long)k * groups * 3 * WAVE + lane;
+  sum3(gb, gb + WAVE, gb + 2 * WAVE, 3 * WAVE, groups, s);
+  const double tE = sqrt(s[0] * ctl.norm.inv_pss);
+  const double tW = sqrt(s[1] * ctl.norm.inv_vss);
+  const double tF = sqrt(s[2] * ctl.norm.inv_fss);
   const int p = k * WAVE + lane;
   if (p < ctl.P) {
     ctl.fitness[p] =
       (float)(ctl.norm.w_energy * tE + ctl.norm.w_stress * tW + ctl.norm.w_force * tF);
-    ctl.terms[3 * (size_t)p + 0] = tE;
-    ctl.terms[3 * (size_t)p + 1] = tW;
-    ctl.terms[3 * (size_t)p + 2] = tF;
+    ctl.terms[3 * (long)p + 0] = tE;
+    ctl.terms[3 * (long)p + 1] = tW;
+    ctl.terms[3 * (long)p + 2] = tF;
   }
 }
 
+// Store a chunk partial and, for the last chunk to arrive, fold all chunk
+// partials of (c, k) in chunk order into acc.  Returns false for the others.
+__device__ inline bool fold_chunks(const PopCtl& ctl, unsigned* counter, int c, int k, int chunk,
+                                   int nch, int lane, Acc& acc)
+{
+  const long slot0 = ldu(ctl.cfg_slot0, c);
+  double* slot = ctl.part + ((slot0 + chunk) * ctl.K + k) * 8 * WAVE;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) slot[u * WAVE + lane] = acc.v[u];
+  __threadfence();
+  const unsigned done = wave_atomic_add(counter, 1u);
+  if (done != (unsigned)nch - 1u) return false;
+  __threadfence();
+#pragma unroll
+  for (int u = 0; u < 8; ++u) acc.v[u] = 0.0;
+  for (int ch = 0; ch < nch; ++ch) {
+    const double* s = ctl.part + ((slot0 + ch) * ctl.K + k) * 8 * WAVE;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc.v[u] += s[u * WAVE + lane];
+  }
+  return true;
+}
+
 // Pot must provide:
-//   struct Lane;                                   per-lane constants
-//   static Lane make(ctl, p);                      denormalised genes of p
-//   typedef Extra;                                 potential-specific tables
-//   static void chunk(set, q, item, ctl, ex, Acc&) process atoms [a0, a1)
-//   static void force_pass(set, ctl, c, k, lane)   -> sum |F-F_ref|^2 over
-//        configuration c; only called when Pot::kForcePass is true
+//   struct Lane; typedef Extra;                     per-lane constants, tables
+//   static Lane make(ctl, p);                       denormalised genes of p
+//   static void chunk(set, q, item, ctl, ex, Acc&)  evaluate atoms [a0, a1)
+//   static constexpr bool kForcePass;               three-body force assembly
+//   static double force_pass(set, ctl, a0, a1, k, lane)
+//                                                   sum |F-F_ref|^2 over [a0, a1)
 template <class Pot>
 __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
                                                          typename Pot::Extra ex)
 {
   const int lane = __lane_id();
-  const int waves_per_block = blockDim.x / WAVE;
-  const unsigned total_waves = gridDim.x * waves_per_block;
+  const unsigned total_waves = gridDim.x * (blockDim.x / WAVE);
   const int Nc = set.Nc;
   for (;;) {
     const unsigned it = wave_atomic_add(ctl.fetch, 1u);
@@ -132,40 +216,47 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
       break;
     }
     const WorkItem item = ldu(ctl.items, it);
-    const int k = item.k;
-    // individual of this lane; lanes past P repeat the last one and write nothing
-    const int p = min(k * WAVE + lane, ctl.P - 1);
-    const typename Pot::Lane q = Pot::make(ctl, p);
-
+    const int c = item.c, k = item.k;
+    const int nch = ldu(ctl.cfg_nchunks, c);
+    const bool is_force = c < set.Nc_force;
+    unsigned* cnt = &ctl.cnt[(long)c * ctl.K + k];
     Acc acc;
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc.v[u] = 0.0;
-    Pot::chunk(set, q, item, ctl, ex, acc);
 
-    const int c = item.c;
-    const int nch = ldu(ctl.cfg_nchunks, c);
-    const bool is_force = c < set.Nc_force;
-    if (nch > 1) {
-      double* slot = ctl.part + ((size_t)(ldu(ctl.cfg_slot0, c) + item.chunk) * ctl.K + k) * 8 * WAVE;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) slot[u * WAVE + lane] = acc.v[u];
+    if (item.phase == 0) {
+      // individual of this lane; lanes past P repeat the last one, write nothing
+      const int p = min(k * WAVE + lane, ctl.P - 1);
+      const typename Pot::Lane q = Pot::make(ctl, p);
+      Pot::chunk(set, q, item, ctl, ex, acc);
+      if (Pot::kForcePass && is_force) {
+        __threadfence(); // publish the per-bond forces of this chunk
+        wave_atomic_add(cnt, 1u);
+        continue;
+      }
+      if (nch > 1) {
+        if (!fold_chunks(ctl, cnt, c, k, item.chunk, nch, lane, acc)) continue;
+        if (lane == 0) *cnt = 0u;
+      }
+    } else {
+      if (!wave_wait_count(cnt, (unsigned)nch)) {
+        if (lane == 0) atomicOr(ctl.status, 1u);
+      }
       __threadfence();
-      const unsigned done = wave_atomic_add(&ctl.cnt[(size_t)c * ctl.K + k], 1u);
-      if (done != (unsigned)nch - 1u) continue;
-      __threadfence();
-      if (lane == 0) ctl.cnt[(size_t)c * ctl.K + k] = 0u;
-      // fold the chunk partials in chunk order
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc.v[u] = 0.0;
-      for (int ch = 0; ch < nch; ++ch) {
-        const double* s = ctl.part + ((size_t)(ldu(ctl.cfg_slot0, c) + ch) * ctl.K + k) * 8 * WAVE;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) acc.v[u] += s[u * WAVE + lane];
+      acc.v[7] = Pot::force_pass(set, ctl, item.a0, item.a1, k, lane);
+      unsigned* cnt2 = &ctl.cnt2[(long)c * ctl.K + k];
+      if (nch > 1) {
+        if (!fold_chunks(ctl, cnt2, c, k, item.chunk, nch, lane, acc)) continue;
+      }
+      if (lane == 0) {
+        *cnt = 0u;
+        *cnt2 = 0u;
       }
     }
+
     double eE = 0.0, eW = 0.0, eF = 0.0;
     if (is_force) {
-      eF = Pot::kForcePass ? Pot::force_pass(set, ctl, c, k, lane) : acc.v[7];
+      eF = acc.v[7];
     } else {
       const double dE = acc.v[0] - (double)ldu(set.pe_ref, c);
       eE = dE * dE;
@@ -175,15 +266,7 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
         eW += dW * dW;
       }
     }
-    ctl.err[((size_t)(k * 3 + 0) * Nc + c) * WAVE + lane] = eE;
-    ctl.err[((size_t)(k * 3 + 1) * Nc + c) * WAVE + lane] = eW;
-    ctl.err[((size_t)(k * 3 + 2) * Nc + c) * WAVE + lane] = eF;
-    __threadfence();
-    const unsigned cdone = wave_atomic_add(&ctl.blk_done[k], 1u);
-    if (cdone != (unsigned)Nc - 1u) continue;
-    __threadfence();
-    if (lane == 0) ctl.blk_done[k] = 0u;
-    finish_block(ctl, Nc, k, lane);
+    config_done(ctl, Nc, c, k, lane, eE, eW, eF);
   }
 }
 

@@ -231,7 +231,8 @@ struct TersoffPop {
   // One atom for 64 individuals: the geometry is wave-uniform (scalar loads
   // from the atom's record), the parameters are per lane; all loop guards are
   // scalar branches.
-  static constexpr int RS = 1 + NMAX + NMAX * NMAX; // record stride in float4
+  // record stride in float4: header, bonds, triplets, reverse-bond indices
+  static constexpr int RS = 1 + NMAX + NMAX * NMAX + NMAX / 4;
 
   template <bool FORCE>
   __device__ static void atom(const DevSet& s, const TLane& q, int i, float* fblk, int lane,
@@ -338,24 +339,31 @@ struct TersoffPop {
     }
   }
 
-  // F_i = sum_j (f_ij - f_ji) from the per-bond forces of every chunk of
-  // configuration c, then sum_i |F_i - F_ref,i|^2 (src/minimal_tersoff.cu:355-383,
+  // F_i = sum_j (f_ij - f_ji) for the atoms [a0, a1) of a force configuration
+  // from the per-bond forces every chunk of the configuration published, then
+  // sum_i |F_i - F_ref,i|^2 (src/minimal_tersoff.cu:355-383,
   // src/fitness.cu:392-435).
-  __device__ static double force_pass(const DevSet& s, const PopCtl& ctl, int c, int k, int lane)
+  __device__ static double force_pass(const DevSet& s, const PopCtl& ctl, int a0, int a1, int k,
+                                      int lane)
   {
     const float* fblk = ctl.bondf + (size_t)k * ctl.B_force * 3 * WAVE + lane;
-    const int a0 = ldu(s.Na_sum, c), a1 = a0 + ldu(s.Na, c);
     const long N = s.N;
     double err = 0.0;
     for (int i = a0; i < a1; ++i) {
-      const int nn = ldu(s.nn, i), b0 = ldu(s.bond0, i);
+      const float4* R = s.rec + (long)i * RS;
+      const int4 hd = ldu(reinterpret_cast<const int4*>(R), 0);
+      const int* rev = reinterpret_cast<const int*>(R + 1 + NMAX + NMAX * NMAX);
+      const int nn = hd.x, b0 = hd.z;
       float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-      for (int u = 0; u < nn; ++u) {
-        const float* a = fblk + (size_t)(b0 + u) * 3 * WAVE;
-        const float* r = fblk + (size_t)ldu(s.rev, b0 + u) * 3 * WAVE;
-        fx += a[0] - r[0];
-        fy += a[WAVE] - r[WAVE];
-        fz += a[2 * WAVE] - r[2 * WAVE];
+#pragma unroll
+      for (int u = 0; u < NMAX; ++u) {
+        if (u < nn) {
+          const float* a = fblk + (size_t)(b0 + u) * 3 * WAVE;
+          const float* r = fblk + (size_t)ldu(rev, u) * 3 * WAVE;
+          fx += a[0] - r[0];
+          fy += a[WAVE] - r[WAVE];
+          fz += a[2 * WAVE] - r[2 * WAVE];
+        }
       }
       const float dx = fx - ldu(s.force_ref, i);
       const float dy = fy - ldu(s.force_ref, i + N);
