@@ -78,6 +78,7 @@ def lib():
             "gpuga_fitness_set_timing": (C.c_int, [vp, C.c_int]),
             "gpuga_fitness_timing": (C.c_int, [vp, dp, P(C.c_long), C.c_int]),
             "gpuga_fitness_device_bytes": (C.c_int, [vp, dp]),
+            "gpuga_fitness_trace": (C.c_int, [vp, C.c_int, fp, dp, ip]),
             "gpuga_ga_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
                                           C.c_uint, P(vp)]),
             "gpuga_ga_destroy": (None, [vp]),
@@ -223,6 +224,18 @@ class Fitness:
         n = C.c_long()
         _check(lib().gpuga_fitness_timing(self._h, C.byref(ms), C.byref(n), int(reset)))
         return ms.value, n.value
+
+    def trace(self, genes):
+        """Work-item timeline of one population generation: array [n_items, 9]
+        of (config, lane block, chunk, atom begin, atom end, phase, wave slot,
+        start ns, end ns)."""
+        g = _f32(genes).reshape(-1, self.V)
+        n = C.c_int(0)
+        _check(lib().gpuga_fitness_trace(self._h, g.shape[0], _ptr(g), None, C.byref(n)))
+        rows = np.zeros((n.value, 9))
+        _check(lib().gpuga_fitness_trace(self._h, g.shape[0], _ptr(g), _ptr(rows, C.c_double),
+                                         C.byref(n)))
+        return rows
 
     def device_bytes(self):
         b = C.c_double()

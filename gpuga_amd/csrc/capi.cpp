@@ -290,6 +290,19 @@ int gpuga_fitness_timing(gpuga_fitness* h, double* kernel_ms, long* launches, in
   });
 }
 
+int gpuga_fitness_trace(gpuga_fitness* h, int P, const float* genes, double* rows, int* n_items)
+{
+  return guarded([&] {
+    need(h, "handle");
+    need(genes, "genes");
+    need(n_items, "n_items");
+    const std::vector<double> r = h->f->trace(P, genes);
+    const int n = (int)(r.size() / 9);
+    if (rows) std::memcpy(rows, r.data(), sizeof(double) * 9 * std::min(n, *n_items));
+    *n_items = n;
+  });
+}
+
 int gpuga_fitness_device_bytes(const gpuga_fitness* h, double* bytes)
 {
   return guarded([&] {

@@ -33,6 +33,7 @@ struct DevSet {
   // independent of the others, so they issue together.
   const float4* rec;
   int rec_nmax;
+  int bond_force;         // bonds of the force configurations (they come first)
   const float* pe_ref;    // [Nc]
   const float* virial_ref;// [6][Nc]
   const float* force_ref; // [3][N]

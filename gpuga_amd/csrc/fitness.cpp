@@ -226,6 +226,8 @@ void Fitness::setup()
       }
       ds_.rec.upload((const float4*)rec.data(), rec.size(), st);
       ds_.view.rec_nmax = M;
+      ds_.view.bond_force =
+        ts_.N_force > 0 ? nl_.start[ts_.N_force - 1] + nl_.NN[ts_.N_force - 1] : 0;
     }
   } else {
     ds_.tri0.upload(std::vector<int>(ts_.N, 0), st);
@@ -346,7 +348,7 @@ size_t Fitness::plan_bytes(int P) const
   const bool tersoff = spec_.type == POT_TERSOFF1 || spec_.type == POT_TERSOFF2;
   size_t b = K * 3 * ts_.Nc * WAVE * sizeof(double);              // err
   b += (size_t)2 * ts_.Nc * K * 8 * WAVE * sizeof(double);         // part (typical)
-  if (tersoff) b += K * B_force * 3 * WAVE * sizeof(float);       // bondf
+  if (tersoff) b += K * (B_force + ts_.N_force) * 3 * WAVE * sizeof(float);       // bondf
   if (spec_.type == POT_NN2B) b += (size_t)3 * opt_.nn2b_grid * K * WAVE * sizeof(float);
   return b;
 }
@@ -427,7 +429,7 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   pl.genes.resize((size_t)P * spec_.V);
   const int B_force =
     ts_.N_force > 0 ? nl_.start[ts_.N_force - 1] + nl_.NN[ts_.N_force - 1] : 0;
-  pl.bondf.resize(tersoff ? std::max<size_t>((size_t)K * B_force * 3 * WAVE, 1) : 1);
+  pl.bondf.resize(tersoff ? std::max<size_t>((size_t)K * (B_force + ts_.N_force) * 3 * WAVE, 1) : 1);
   if (spec_.type == POT_NN2B) pl.nn2b_table.resize((size_t)K * nn2b_grid_desc().G * 3 * WAVE);
   const int groups = (Nc + CFG_GROUP - 1) / CFG_GROUP;
   pl.gpart.resize((size_t)K * groups * 3 * WAVE);
@@ -454,7 +456,7 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   c.pmax = d_pmax_.get();
   c.part = pl.part.get();
   c.bondf = pl.bondf.get();
-  c.B_force = B_force;
+  c.B_force = B_force + ts_.N_force; // slots per lane block: bonds, then atoms
   c.err = pl.err.get();
   unsigned* ctr = pl.counters.get();
   c.cnt = ctr;
@@ -692,10 +694,43 @@ size_t Fitness::estimate_bytes(const PotentialSpec& s, const TrainSet& ts, int P
   const size_t K = (P + WAVE - 1) / WAVE;
   const size_t B_force = ts.N_force > 0 ? (size_t)nl.start[ts.N_force - 1] + nl.NN[ts.N_force - 1] : 0;
   b += K * 3 * Nc * WAVE * sizeof(double) + 2 * Nc * K * 8 * WAVE * sizeof(double);
-  if (tersoff) b += K * B_force * 3 * WAVE * sizeof(float);
+  if (tersoff) b += K * (B_force + ts.N_force) * 3 * WAVE * sizeof(float);
   if (s.type == POT_NN2B) b += (size_t)3 * opt.nn2b_grid * K * WAVE * sizeof(float);
   b += (size_t)P * s.V * sizeof(float) + (size_t)P * 4 * sizeof(double);
   return b;
+}
+
+std::vector<double> Fitness::trace(int P, const float* genes)
+{
+  PopPlan& pl = plan(P);
+  hipStream_t st = stream_;
+  const size_t n = pl.n_items;
+  DeviceBuffer<unsigned long long> buf(4 * n);
+  buf.zero(st);
+  std::vector<WorkItem> items(n);
+  pl.items.download(items.data(), n, st);
+  GPUGA_HIP(hipMemcpyAsync(pl.genes.get(), genes, sizeof(float) * (size_t)P * spec_.V,
+                           hipMemcpyHostToDevice, st));
+  pl.ctl.trace = buf.get();
+  launch_population(pl, st);
+  pl.ctl.trace = nullptr;
+  std::vector<unsigned long long> t(4 * n);
+  buf.download(t.data(), t.size(), st);
+  GPUGA_HIP(hipStreamSynchronize(st));
+  int khz = 0;
+  GPUGA_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_));
+  const double ns_per_tick = 1e6 / std::max(khz, 1);
+  unsigned long long t0 = ~0ull;
+  for (size_t i = 0; i < n; ++i) t0 = std::min(t0, t[4 * i + 1]);
+  std::vector<double> rows;
+  rows.reserve(9 * n);
+  for (size_t i = 0; i < n; ++i) {
+    const WorkItem& w = items[i];
+    rows.insert(rows.end(), {(double)w.c, (double)w.k, (double)w.chunk, (double)w.a0,
+                             (double)w.a1, (double)w.phase, (double)t[4 * i],
+                             (t[4 * i + 1] - t0) * ns_per_tick, (t[4 * i + 2] - t0) * ns_per_tick});
+  }
+  return rows;
 }
 
 double Fitness::time_plan(int P, int chunk_cost, int reps)

@@ -82,6 +82,10 @@ public:
   void compute_wait(float* fitness, double* terms);
   // Tune the work-item size for population size P.
   int autotune(int P);
+  // One generation with the item timeline recorded: rows of 9 values per
+  // work item (config, lane block, chunk, atom begin, atom end, phase, wave
+  // slot, start ns, end ns; times relative to the first start).
+  std::vector<double> trace(int P, const float* genes);
   // Work-item size from the training set alone (no timing; deterministic).
   int heuristic_chunk_cost(int P) const;
   // Device bytes a handle for this training set and population would hold.

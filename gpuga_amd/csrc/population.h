@@ -73,6 +73,9 @@ struct PopCtl {
   float* fitness;         // [P]
   double* terms;          // [P][3]
   FitnessNorm norm;
+  // optional item timeline: [n_items][4] = wave slot, start, end, 0 in
+  // wall-clock ticks (constant rate); null when not recording
+  unsigned long long* trace;
 };
 
 // Partial sums carried by one lane over a chunk: E, W[6], sum |F - F_ref|^2.
@@ -207,9 +210,20 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
 {
   const int lane = __lane_id();
   const unsigned total_waves = gridDim.x * (blockDim.x / WAVE);
+  const unsigned wave_slot = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
   const int Nc = set.Nc;
+  unsigned prev = 0xffffffffu;
   for (;;) {
     const unsigned it = wave_atomic_add(ctl.fetch, 1u);
+    if (ctl.trace && lane == 0) {
+      const unsigned long long now = wall_clock64();
+      if (prev != 0xffffffffu) ctl.trace[4 * (long)prev + 2] = now;
+      if (it < (unsigned)ctl.n_items) {
+        ctl.trace[4 * (long)it + 0] = wave_slot;
+        ctl.trace[4 * (long)it + 1] = now;
+      }
+      prev = it;
+    }
     if (it >= (unsigned)ctl.n_items) {
       // The last fetch of the launch returns the counter to zero.
       if (it == (unsigned)ctl.n_items + total_waves - 1u && lane == 0) *ctl.fetch = 0u;

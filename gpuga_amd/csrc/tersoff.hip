@@ -300,10 +300,16 @@ struct TersoffPop {
         const float fy = coef * r.y - hc * cy;
         const float fz = coef * r.z - hc * cz;
         if (FORCE) {
-          float* o = fblk + (size_t)(b0 + u) * 3 * WAVE + lane;
+          // f_ij goes to the slot of the reverse bond, where atom j reads it
+          // back as f_ji; F_i's own part sum_j f_ij stays in registers
+          const int rb = ldu(reinterpret_cast<const int*>(R + 1 + NMAX + NMAX * NMAX), u);
+          float* o = fblk + (size_t)rb * 3 * WAVE + lane;
           o[0] = fx;
           o[WAVE] = fy;
           o[2 * WAVE] = fz;
+          wxx += fx;
+          wyy += fy;
+          wzz += fz;
         } else {
           // total virial of the configuration: sum_i W_i = -sum_ij r_ij (x) f_ij
           // because every bond's reverse carries -r_ij (exact FP32 minimum image)
@@ -316,7 +322,13 @@ struct TersoffPop {
         }
       }
     }
-    if (!FORCE) {
+    if (FORCE) {
+      // own part of F_i, in the per-atom slots after the B_force bond slots
+      float* o = fblk + ((size_t)s.bond_force + i) * 3 * WAVE + lane;
+      o[0] = wxx;
+      o[WAVE] = wyy;
+      o[2 * WAVE] = wzz;
+    } else {
       acc.v[0] += (double)(0.5f * e);
       acc.v[1] += (double)wxx;
       acc.v[2] += (double)wyy;
@@ -346,29 +358,37 @@ struct TersoffPop {
   __device__ static double force_pass(const DevSet& s, const PopCtl& ctl, int a0, int a1, int k,
                                       int lane)
   {
+    // per lane block: bond_force slots of received f_ji, then one slot per
+    // force-configuration atom with its own sum_j f_ij (written in phase 0)
     const float* fblk = ctl.bondf + (size_t)k * ctl.B_force * 3 * WAVE + lane;
     const long N = s.N;
     double err = 0.0;
-    for (int i = a0; i < a1; ++i) {
-      const float4* R = s.rec + (long)i * RS;
-      const int4 hd = ldu(reinterpret_cast<const int4*>(R), 0);
-      const int* rev = reinterpret_cast<const int*>(R + 1 + NMAX + NMAX * NMAX);
-      const int nn = hd.x, b0 = hd.z;
-      float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    // two atoms per step, every load unconditional, so all of them are in
+    // flight together
+    for (int i = a0; i < a1; i += 2) {
+      float d[2][3];
 #pragma unroll
-      for (int u = 0; u < NMAX; ++u) {
-        if (u < nn) {
-          const float* a = fblk + (size_t)(b0 + u) * 3 * WAVE;
-          const float* r = fblk + (size_t)ldu(rev, u) * 3 * WAVE;
-          fx += a[0] - r[0];
-          fy += a[WAVE] - r[WAVE];
-          fz += a[2 * WAVE] - r[2 * WAVE];
+      for (int h = 0; h < 2; ++h) {
+        const int ii = min(i + h, a1 - 1);
+        const int4 hd = ldu(reinterpret_cast<const int4*>(s.rec + (long)ii * RS), 0);
+        const float* own = fblk + ((size_t)s.bond_force + ii) * 3 * WAVE;
+        float fx = own[0], fy = own[WAVE], fz = own[2 * WAVE];
+#pragma unroll
+        for (int u = 0; u < NMAX; ++u) {
+          const bool on = u < hd.x;
+          const float* r = fblk + (size_t)(on ? hd.z + u : hd.z) * 3 * WAVE;
+          const float rx = r[0], ry = r[WAVE], rz = r[2 * WAVE];
+          fx -= on ? rx : 0.0f;
+          fy -= on ? ry : 0.0f;
+          fz -= on ? rz : 0.0f;
         }
+        d[h][0] = fx - ldu(s.force_ref, ii);
+        d[h][1] = fy - ldu(s.force_ref, ii + N);
+        d[h][2] = fz - ldu(s.force_ref, ii + 2 * N);
       }
-      const float dx = fx - ldu(s.force_ref, i);
-      const float dy = fy - ldu(s.force_ref, i + N);
-      const float dz = fz - ldu(s.force_ref, i + 2 * N);
-      err += (double)dx * dx + (double)dy * dy + (double)dz * dz;
+      err += (double)d[0][0] * d[0][0] + (double)d[0][1] * d[0][1] + (double)d[0][2] * d[0][2];
+      if (i + 1 < a1)
+        err += (double)d[1][0] * d[1][0] + (double)d[1][1] * d[1][1] + (double)d[1][2] * d[1][2];
     }
     return err;
   }

@@ -103,6 +103,12 @@ int gpuga_fitness_autotune(gpuga_fitness* h, int P, int* chunk_cost);
 int gpuga_fitness_set_timing(gpuga_fitness* h, int on);
 int gpuga_fitness_timing(gpuga_fitness* h, double* kernel_ms, long* launches, int reset);
 
+/* One population generation with its work-item timeline recorded (no
+ * reference counterpart; a profiling aid).  rows[n_items][9] = configuration,
+ * lane block, chunk, first atom, end atom, phase, wave slot, start ns, end ns
+ * (relative to the first start).  Call with rows = NULL to get *n_items. */
+int gpuga_fitness_trace(gpuga_fitness* h, int P, const float* genes, double* rows, int* n_items);
+
 /* Device bytes held by the handle (training set, tables, plans). */
 int gpuga_fitness_device_bytes(const gpuga_fitness* h, double* bytes);
 
