@@ -83,6 +83,28 @@ struct Acc {
   double v[8];
 };
 
+// Cross-wave data (chunk partials, error terms, per-bond forces) is written
+// and read with device-scope relaxed atomics: the accesses are coherent
+// across the XCDs (they do not rest in an XCD's L2), so no cache-wide write
+// back or invalidate is ever needed.  publish() waits until every such store
+// of the wave has completed before the counter update that announces it;
+// observe() keeps the compiler from moving the reads before the counter
+// check that guards them (the hardware issues them after it).
+template <class T>
+__device__ __forceinline__ void cstore(T* p, T v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class T>
+__device__ __forceinline__ T cload(const T* p)
+{
+  return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void publish() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ __forceinline__ void observe() { asm volatile("" ::: "memory"); }
+
 __device__ inline unsigned wave_atomic_add(unsigned* p, unsigned x)
 {
   unsigned old = 0;
@@ -95,9 +117,12 @@ __device__ inline bool wave_wait_count(const unsigned* p, unsigned target)
 {
   for (int spin = 0; spin < (1 << 23); ++spin) {
     unsigned v = 0;
-    if (__lane_id() == 0) v = __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if (__lane_id() == 0) v = cload(p);
     v = (unsigned)wave_first((int)v);
-    if (v >= target) return true;
+    if (v >= target) {
+      observe();
+      return true;
+    }
     __builtin_amdgcn_s_sleep(8);
   }
   return false;
@@ -113,15 +138,15 @@ __device__ inline void sum3(const double* e0, const double* e1, const double* e2
   for (; c + 4 <= n; c += 4) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      s0[u] += e0[(c + u) * stride];
-      s1[u] += e1[(c + u) * stride];
-      s2[u] += e2[(c + u) * stride];
+      s0[u] += cload(e0 + (c + u) * stride);
+      s1[u] += cload(e1 + (c + u) * stride);
+      s2[u] += cload(e2 + (c + u) * stride);
     }
   }
   for (; c < n; ++c) {
-    s0[0] += e0[c * stride];
-    s1[0] += e1[c * stride];
-    s2[0] += e2[c * stride];
+    s0[0] += cload(e0 + c * stride);
+    s1[0] += cload(e1 + c * stride);
+    s2[0] += cload(e2 + c * stride);
   }
   out[0] = (s0[0] + s0[1]) + (s0[2] + s0[3]);
   out[1] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
@@ -136,28 +161,28 @@ __device__ inline void config_done(const PopCtl& ctl, int Nc, int c, int k, int 
 {
   const long E3 = (long)Nc * WAVE;
   double* e = ctl.err + (long)k * 3 * E3 + (long)c * WAVE + lane;
-  e[0] = eE;
-  e[E3] = eW;
-  e[2 * E3] = eF;
+  cstore(e, eE);
+  cstore(e + E3, eW);
+  cstore(e + 2 * E3, eF);
   const int groups = (Nc + CFG_GROUP - 1) / CFG_GROUP;
   const int g = c / CFG_GROUP;
   const int g0 = g * CFG_GROUP, gn = min(CFG_GROUP, Nc - g0);
-  __threadfence();
+  publish();
   const unsigned gd = wave_atomic_add(&ctl.grp_done[k * groups + g], 1u);
   if (gd != (unsigned)gn - 1u) return;
-  __threadfence();
+  observe();
   if (lane == 0) ctl.grp_done[k * groups + g] = 0u;
   const double* b = ctl.err + (long)k * 3 * E3 + (long)g0 * WAVE + lane;
   double s[3];
   sum3(b, b + E3, b + 2 * E3, WAVE, gn, s);
   double* gp = ctl.gpart + ((long)k * groups + g) * 3 * WAVE + lane;
-  gp[0] = s[0];
-  gp[WAVE] = s[1];
-  gp[2 * WAVE] = s[2];
-  __threadfence();
+  cstore(gp, s[0]);
+  cstore(gp + WAVE, s[1]);
+  cstore(gp + 2 * WAVE, s[2]);
+  publish();
   const unsigned bd = wave_atomic_add(&ctl.blk_done[k], 1u);
   if (bd != (unsigned)groups - 1u) return;
-  __threadfence();
+  observe();
   if (lane == 0) ctl.blk_done[k] = 0u;
   const double* gb = ctl.gpart + (long)k * groups * 3 * WAVE + lane;
   sum3(gb, gb + WAVE, gb + 2 * WAVE, 3 * WAVE, groups, s);
@@ -182,17 +207,17 @@ __device__ inline bool fold_chunks(const PopCtl& ctl, unsigned* counter, int c, 
   const long slot0 = ldu(ctl.cfg_slot0, c);
   double* slot = ctl.part + ((slot0 + chunk) * ctl.K + k) * 8 * WAVE;
 #pragma unroll
-  for (int u = 0; u < 8; ++u) slot[u * WAVE + lane] = acc.v[u];
-  __threadfence();
+  for (int u = 0; u < 8; ++u) cstore(slot + u * WAVE + lane, acc.v[u]);
+  publish();
   const unsigned done = wave_atomic_add(counter, 1u);
   if (done != (unsigned)nch - 1u) return false;
-  __threadfence();
+  observe();
 #pragma unroll
   for (int u = 0; u < 8; ++u) acc.v[u] = 0.0;
   for (int ch = 0; ch < nch; ++ch) {
     const double* s = ctl.part + ((slot0 + ch) * ctl.K + k) * 8 * WAVE;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc.v[u] += s[u * WAVE + lane];
+    for (int u = 0; u < 8; ++u) acc.v[u] += cload(s + u * WAVE + lane);
   }
   return true;
 }
@@ -244,7 +269,7 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
       const typename Pot::Lane q = Pot::make(ctl, p);
       Pot::chunk(set, q, item, ctl, ex, acc);
       if (Pot::kForcePass && is_force) {
-        __threadfence(); // publish the per-bond forces of this chunk
+        publish(); // the per-bond forces of this chunk are complete
         wave_atomic_add(cnt, 1u);
         continue;
       }
@@ -256,7 +281,6 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
       if (!wave_wait_count(cnt, (unsigned)nch)) {
         if (lane == 0) atomicOr(ctl.status, 1u);
       }
-      __threadfence();
       acc.v[7] = Pot::force_pass(set, ctl, item.a0, item.a1, k, lane);
       unsigned* cnt2 = &ctl.cnt2[(long)c * ctl.K + k];
       if (nch > 1) {

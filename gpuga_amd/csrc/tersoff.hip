@@ -304,9 +304,9 @@ struct TersoffPop {
           // back as f_ji; F_i's own part sum_j f_ij stays in registers
           const int rb = ldu(reinterpret_cast<const int*>(R + 1 + NMAX + NMAX * NMAX), u);
           float* o = fblk + (size_t)rb * 3 * WAVE + lane;
-          o[0] = fx;
-          o[WAVE] = fy;
-          o[2 * WAVE] = fz;
+          cstore(o, fx);
+          cstore(o + WAVE, fy);
+          cstore(o + 2 * WAVE, fz);
           wxx += fx;
           wyy += fy;
           wzz += fz;
@@ -325,9 +325,9 @@ struct TersoffPop {
     if (FORCE) {
       // own part of F_i, in the per-atom slots after the B_force bond slots
       float* o = fblk + ((size_t)s.bond_force + i) * 3 * WAVE + lane;
-      o[0] = wxx;
-      o[WAVE] = wyy;
-      o[2 * WAVE] = wzz;
+      cstore(o, wxx);
+      cstore(o + WAVE, wyy);
+      cstore(o + 2 * WAVE, wzz);
     } else {
       acc.v[0] += (double)(0.5f * e);
       acc.v[1] += (double)wxx;
@@ -372,12 +372,12 @@ struct TersoffPop {
         const int ii = min(i + h, a1 - 1);
         const int4 hd = ldu(reinterpret_cast<const int4*>(s.rec + (long)ii * RS), 0);
         const float* own = fblk + ((size_t)s.bond_force + ii) * 3 * WAVE;
-        float fx = own[0], fy = own[WAVE], fz = own[2 * WAVE];
+        float fx = cload(own), fy = cload(own + WAVE), fz = cload(own + 2 * WAVE);
 #pragma unroll
         for (int u = 0; u < NMAX; ++u) {
           const bool on = u < hd.x;
           const float* r = fblk + (size_t)(on ? hd.z + u : hd.z) * 3 * WAVE;
-          const float rx = r[0], ry = r[WAVE], rz = r[2 * WAVE];
+          const float rx = cload(r), ry = cload(r + WAVE), rz = cload(r + 2 * WAVE);
           fx -= on ? rx : 0.0f;
           fy -= on ? ry : 0.0f;
           fz -= on ? rz : 0.0f;

@@ -141,3 +141,24 @@ def test_chunking_does_not_change_results(G, golden):
     a, ta = small.compute(genes, terms=True)
     b, tb = whole.compute(genes, terms=True)
     np.testing.assert_allclose(ta, tb, rtol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["si_diamond", "gaas", "hfo2"])
+def test_cross_wave_handoff_stress(G, golden, name):
+    """Tiny work items split every configuration into many chunks, so chunk
+    partials, per-bond forces and error terms pass between waves on different
+    XCDs many times per launch; ten launches must reproduce the single-item
+    result (FP64 partial sums regrouped: 1e-12)."""
+    print_case("handoff stress", name)
+    V = G.Fitness(golden(name), chunk_cost=10 ** 8).V
+    genes = genes_for(V, 130, 77)
+    whole = G.Fitness(golden(name), chunk_cost=10 ** 8)
+    ref_fit, ref_terms = whole.compute(genes, terms=True)
+    tiny = G.Fitness(golden(name), chunk_cost=24, use_graph=1)
+    first = None
+    for rep in range(10):
+        f, t = tiny.compute(genes, terms=True)
+        np.testing.assert_allclose(t, ref_terms, rtol=1e-12)
+        if first is None:
+            first = t
+        assert np.array_equal(t, first)
