@@ -391,8 +391,12 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
       acc += ci;
       cacc += ci;
       mx = std::max(mx, nl_.NN[i]);
-      const bool cut = (k < n - 1) && (acc * n >= total * (k + 1)) && (a1 - (i + 1) >= n - 1 - k);
-      if (cut || i == a1 - 1) {
+      // cut at the k-th cost quantile, or when the atoms left are exactly
+      // the chunks still to make
+      const int left_atoms = a1 - (i + 1), left_chunks = n - 1 - k;
+      const bool must = left_chunks > 0 && left_atoms == left_chunks;
+      const bool want = left_chunks > 0 && left_atoms >= left_chunks && acc * n >= total * (k + 1);
+      if (must || want || i == a1 - 1) {
         chunks.push_back(Chunk{c, k, start, i + 1, nn_class(mx), cacc});
         start = i + 1;
         ++k;
