@@ -18,7 +18,7 @@ struct FitnessOptions {
   int chunk_cost = 0;     // work-item cost target; 0 = autotune on first use
   int use_graph = 1;      // replay each generation from a captured HIP graph
   int nn2b_bf16 = 1;      // NN2B hidden layer 2 on BF16 MFMA (else FP32)
-  int nn2b_grid = 4096;   // NN2B tabulation points
+  int nn2b_grid = 1024;   // NN2B tabulation points
   int verbose = 0;
   int blocks_per_cu = 0;  // 0 = from the occupancy calculator
 };

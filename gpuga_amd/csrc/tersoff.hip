@@ -240,7 +240,7 @@ struct TersoffPop {
   {
     const float4* R = s.rec + (long)i * RS;
     const int4 hd = ldu(reinterpret_cast<const int4*>(R), 0);
-    const int nn = hd.x, unl = hd.y, b0 = hd.z;
+    const int nn = hd.x, unl = hd.y;
     float fc[NMAX], fcp[NMAX], fcb[NMAX], A[NMAX], B[NMAX], pf[NMAX];
 #pragma unroll
     for (int u = 0; u < NMAX; ++u) {

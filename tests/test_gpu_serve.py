@@ -1,0 +1,72 @@
+"""Serving loop: concurrent fitting jobs, each with its own resident training
+set, population and stream.  A job's results (best fitness, elite, ga.out)
+must be the same run alone and run together with other jobs, and the
+memory budget must hold jobs back until memory frees."""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+
+def job_dir(golden, tmp, name, gens, P, parents, seed):
+    d = os.path.join(tmp, f"{name}_{seed}")
+    if os.path.isdir(d):
+        shutil.rmtree(d)
+    shutil.copytree(golden(name), d)
+    with open(os.path.join(d, "ga.in"), "w") as f:
+        f.write(f"maximum_generation {gens}\npopulation_size {P}\nparent_number {parents}\n"
+                f"mutation_rate 0.2\nseed {seed}\n")
+    return d
+
+
+JOBS = [("si_diamond", 6, 64, 32, 1), ("gaas", 5, 40, 20, 2), ("nn2b", 6, 70, 36, 3),
+        ("si_example", 4, 130, 64, 4)]
+
+
+def run_jobs(G, dirs, budget=64e9, max_active=8):
+    s = G.Server(memory_budget_bytes=budget, max_active=max_active, nn2b_bf16=0)
+    ids = [s.submit(d, write_outputs=True) for d in dirs]
+    s.drain()
+    return s, ids
+
+
+def test_results_same_alone_and_in_the_mix(golden, tmp_path):
+    import gpuga_amd as G
+    print_case("serve alone vs mix")
+    alone, mixed = {}, {}
+    for name, gens, P, par, seed in JOBS:
+        d = job_dir(golden, str(tmp_path / "alone"), name, gens, P, par, seed)
+        s, ids = run_jobs(G, [d])
+        info = s.job(ids[0], 2000)
+        alone[name] = (info, open(os.path.join(d, "ga.out")).read())
+        assert info["state"] == 2 and info["generations"] == gens
+    dirs = [job_dir(golden, str(tmp_path / "mix"), *j) for j in JOBS]
+    s, ids = run_jobs(G, dirs)
+    for (name, gens, *_), jid, d in zip(JOBS, ids, dirs):
+        info = s.job(jid, 2000)
+        assert info["state"] == 2
+        assert info["best"] == alone[name][0]["best"]
+        assert np.array_equal(info["elite"], alone[name][0]["elite"])
+        assert open(os.path.join(d, "ga.out")).read() == alone[name][1]
+        for out in ("force.out", "energy.out", "virial.out"):
+            assert os.path.getsize(os.path.join(d, out)) > 0
+    lat = s.latencies()
+    assert len(lat) == sum(j[1] for j in JOBS)
+    assert np.all(lat > 0)
+
+
+def test_memory_budget_serialises_jobs(golden, tmp_path):
+    import gpuga_amd as G
+    print_case("serve budget")
+    dirs = [job_dir(golden, str(tmp_path), "si_diamond", 3, 64, 32, s) for s in (11, 12, 13)]
+    s = G.Server(memory_budget_bytes=1.0, max_active=8)  # admits one job at a time
+    ids = [s.submit(d) for d in dirs]
+    a, q, f = s.poll()
+    assert a == 1 and q == 2
+    s.drain()
+    assert all(s.job(i, 9)["state"] == 2 for i in ids)
