@@ -134,8 +134,10 @@ def serve(args):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(d, w, genes, budget_s=15.0):
-    """FP64 oracle on the same training set, a bounded sample of individuals."""
+def cpu_baseline(d, w, genes, budget_s=12.0):
+    """FP64 oracle on the same training set, a bounded sample of individuals:
+    one thread (the reported value), then OpenMP over individuals with one
+    thread per host core of this process's share (at most 16)."""
     from oracle import oracle as O
     case = O.Case(potential=dict(potential_type=d["potential_type"], cutoff=d["cutoff"],
                                  weights=d["weights"], pmin=d["pmin"], pmax=d["pmax"],
@@ -149,9 +151,31 @@ def cpu_baseline(d, w, genes, budget_s=15.0):
     t0 = time.perf_counter()
     case.fitness(genes[:n], threads=1)
     dt = time.perf_counter() - t0
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    m = max(threads, min(len(genes), threads * n))
+    t0 = time.perf_counter()
+    case.fitness(genes[:m], threads=threads)
+    dtm = time.perf_counter() - t0
     return {"value": n * N / dt, "unit": "individual*atom evals/s", "cores": 1, "kind": "port",
             "sample": f"{n} individuals x {N} atoms (full training set), FP64 scalar C oracle, "
-                      f"1 thread, {dt:.1f} s"}
+                      f"1 thread, {dt:.1f} s",
+            "openmp": {"value": m * N / dtm, "cores": threads,
+                       "sample": f"{m} individuals x {N} atoms, {threads} threads, {dtm:.1f} s"}}
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the dominant kernel from the committed
+    FETCH_SIZE / WRITE_SIZE profile of this workload (each counter scaled by
+    what it reads for a plain copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+    if not t:
+        return None
+    return 1024.0 * (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"])
 
 
 def main():
@@ -187,7 +211,8 @@ def main():
     chunk = args.chunk_cost
     if not args.plain and chunk == 0:
         chunk = fit.autotune(P)
-    ga = gpuga_amd.GA(P, P // 2, V, maximum_generation=args.warmup + args.steps,
+    parents = P // 2 + (P - P // 2) % 2  # the GA pairs the children: P - parents even
+    ga = gpuga_amd.GA(P, parents, V, maximum_generation=args.warmup + args.steps,
                       mutation_rate=0.2, seed=12345678)
 
     def step(g):
@@ -237,11 +262,12 @@ def main():
                    "chunk_cost": chunk, "s_per_generation": ms_step / 1e3,
                    "best_fitness": best},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": MI355X_FP32_PEAK / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / MI355X_FP32_PEAK, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / MI355X_FP32_PEAK,
+                     "traffic": None if args.plain else measured_traffic(args.workload),
                      "kernel_ms": kernel_s * 1e3},
     }
     if not args.no_cpu_baseline and not args.plain:
-        genes = synth.random_genes(64, V, seed=99)
+        genes = synth.random_genes(256, V, seed=99)
         out["cpu_baseline"] = cpu_baseline(d, w, genes)
     print(json.dumps(out), flush=True)
 

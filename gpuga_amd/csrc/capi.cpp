@@ -50,6 +50,8 @@ FitnessOptions to_options(const gpuga_options* o)
   f.nn2b_bf16 = o->nn2b_bf16;
   if (o->nn2b_grid > 0) f.nn2b_grid = o->nn2b_grid;
   f.verbose = o->verbose;
+  f.blocks_per_cu = o->blocks_per_cu;
+  f.pack_lanes = o->pack_lanes;
   return f;
 }
 
@@ -74,6 +76,8 @@ void gpuga_default_options(gpuga_options* opt)
   opt->nn2b_bf16 = f.nn2b_bf16;
   opt->nn2b_grid = f.nn2b_grid;
   opt->verbose = f.verbose;
+  opt->blocks_per_cu = f.blocks_per_cu;
+  opt->pack_lanes = f.pack_lanes;
 }
 
 int gpuga_fitness_create(const char* input_dir, const gpuga_options* opt, gpuga_fitness** out)

@@ -26,14 +26,25 @@ struct DevSet {
   const float4* bond;     // [B] r_ij.x, r_ij.y, r_ij.z, |r_ij|
   const int* tri0;        // [N] first triplet of atom i (Tersoff only)
   const float4* tri;      // [T] cos theta, d cos / d r_ij (Tersoff only)
-  // Tersoff atom records, one per atom at a fixed stride of 1 + M + M*M
-  // float4 (M = neighbour bound of the set): header {nn, unlike-type bit
+  // Tersoff atom records, one per atom at a fixed stride of 1 + M + M*M +
+  // M/2 float4 (M = neighbour bound of the set): header {nn, unlike-type bit
   // mask, first bond, 0}, then M bonds (x, y, z, d), then the M x M
-  // triplets (cos theta_jik, d cos / d r_ij).  Every load of an atom is
-  // independent of the others, so they issue together.
+  // triplets (cos theta_jik, d cos / d r_ij), then M reverse-bond indices
+  // (int) and M inverse distances 1/d.  Every load of an atom is independent
+  // of the others, so they issue together.
   const float4* rec;
   int rec_nmax;
   int bond_force;         // bonds of the force configurations (they come first)
+  // Pair-potential population tables.  An atom's pairs are all its bonds in
+  // a force configuration and those with j > i otherwise.
+  //  RI:   per atom {first pair, pairs of type 0, 1, 2} (grouped by type
+  //        t_i + t_j); per pair two float4: (unit vector r_ij / d, d) and
+  //        (d^-6, 6 d^-7, erfc(a d)/d, -(d/dr)(erfc(a d)/d)).
+  //  NN2B: per atom {first pair, pairs, 0, 0} (ordered by d); per pair three
+  //        float4: (unit vector, d), cubic Hermite weights on the tabulation
+  //        grid, (fc, fc', grid cell as int bits, 0).
+  const int4* pair_hdr;
+  const float4* pair_tab;
   const float* pe_ref;    // [Nc]
   const float* virial_ref;// [6][Nc]
   const float* force_ref; // [3][N]

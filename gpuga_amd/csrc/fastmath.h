@@ -27,6 +27,14 @@ __device__ __forceinline__ T ldu(const T* p, long i)
 #endif
 }
 
+// ldu() when the index is wave-uniform (U), an ordinary per-lane load otherwise.
+template <bool U, class T>
+__device__ __forceinline__ T ldx(const T* p, long i)
+{
+  if constexpr (U) return ldu(p, i);
+  else return p[i];
+}
+
 constexpr float LOG2E = 1.4426950408889634f;
 constexpr float PI_F = 3.14159265358979f;
 

@@ -8,7 +8,7 @@
 
 namespace gpuga {
 
-int tersoff_population_occupancy(int cls, bool two, int block);
+int tersoff_population_occupancy(int cls, bool two, bool pack, int block);
 int ri_population_occupancy(int block);
 int nn2b_population_occupancy(int block);
 
@@ -119,7 +119,8 @@ size_t DeviceSet::bytes() const
 {
   return Na.bytes() + Na_sum.bytes() + type.bytes() + nn.bytes() + bond0.bytes() + nl.bytes() +
          rev.bytes() + ptype.bytes() + tri0.bytes() + bond.bytes() + tri.bytes() +
-         ri_tab.bytes() + rec.bytes() + pe_ref.bytes() + virial_ref.bytes() + force_ref.bytes();
+         ri_tab.bytes() + rec.bytes() + pair_hdr.bytes() + pair_tab.bytes() + pe_ref.bytes() +
+         virial_ref.bytes() + force_ref.bytes();
 }
 
 PopPlan::~PopPlan()
@@ -148,6 +149,9 @@ Fitness::~Fitness()
   plans_.clear();
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
+  if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+  if (ev_join_) (void)hipEventDestroy(ev_join_);
+  if (side_) (void)hipStreamDestroy(side_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -161,8 +165,11 @@ void Fitness::setup()
   GPUGA_HIP(hipGetDeviceProperties(&prop, device_));
   num_cu_ = prop.multiProcessorCount;
   GPUGA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  GPUGA_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
   GPUGA_HIP(hipEventCreate(&ev0_));
   GPUGA_HIP(hipEventCreate(&ev1_));
+  GPUGA_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+  GPUGA_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
 
   const bool tersoff = spec_.type == POT_TERSOFF1 || spec_.type == POT_TERSOFF2;
   if (spec_.type == POT_RI)
@@ -208,7 +215,7 @@ void Fitness::setup()
     if (nl_.max_nn <= 16) {
       // fixed-stride atom records for the population kernel (see DevSet::rec)
       const int M = nl_.max_nn <= 4 ? 4 : (nl_.max_nn <= 8 ? 8 : 16);
-      const size_t RS = 1 + M + (size_t)M * M + M / 4;
+      const size_t RS = 1 + M + (size_t)M * M + M / 2;
       std::vector<Vec4> rec((size_t)ts_.N * RS, Vec4{0, 0, 0, 0});
       for (int i = 0; i < ts_.N; ++i) {
         const int nn = nl_.NN[i], b0 = nl_.start[i];
@@ -223,6 +230,8 @@ void Fitness::setup()
         std::memcpy(&R[0], hdr, sizeof hdr);
         int* rv = reinterpret_cast<int*>(&R[1 + M + M * M]);
         for (int u = 0; u < nn; ++u) rv[u] = nl_.rev[b0 + u];
+        float* inv = reinterpret_cast<float*>(&R[1 + M + M * M + M / 4]);
+        for (int u = 0; u < nn; ++u) inv[u] = (float)(1.0 / (double)geo_.bond[b0 + u].w);
       }
       ds_.rec.upload((const float4*)rec.data(), rec.size(), st);
       ds_.view.rec_nmax = M;
@@ -244,6 +253,82 @@ void Fitness::setup()
       tab[b] = make_float4((float)std::pow(dinv, 6), (float)e, (float)g, (float)dinv);
     }
     ds_.ri_tab.upload(tab, st);
+    // population form: per atom the pairs it evaluates (all bonds for force
+    // configurations, those with j > i for the others), grouped by pair type
+    // t_i + t_j, each as unit vector and d, then the factors plus 6 d^-7
+    std::vector<float4> pt;
+    pt.reserve(2 * (size_t)nl_.total);
+    std::vector<int4> hdr(std::max(ts_.N, 1));
+    for (int i = 0; i < ts_.N; ++i) {
+      const int nn = nl_.NN[i], b0 = nl_.start[i];
+      const bool all = i < ts_.N_force;
+      int cnt[3] = {0, 0, 0};
+      const int start = (int)(pt.size() / 2);
+      for (int t = 0; t < 3; ++t)
+        for (int u = 0; u < nn; ++u) {
+          const int b = b0 + u;
+          if (ptype[b] != t || (!all && nl_.NL[b] < i)) continue;
+          const Vec4 r = geo_.bond[b];
+          const double dinv = 1.0 / (double)r.w;
+          const double d6 = std::pow(dinv, 6);
+          pt.push_back(make_float4((float)(r.x * dinv), (float)(r.y * dinv),
+                                   (float)(r.z * dinv), r.w));
+          pt.push_back(make_float4((float)d6, (float)(6.0 * d6 * dinv), tab[b].y, tab[b].z));
+          ++cnt[t];
+        }
+      hdr[i] = make_int4(start, cnt[0], cnt[1], cnt[2]);
+    }
+    if (pt.empty()) pt.resize(2);
+    ds_.pair_tab.upload(pt, st);
+    ds_.pair_hdr.upload(hdr, st);
+  }
+  if (spec_.type == POT_NN2B) {
+    // population form: per atom the pairs it evaluates (all bonds for force
+    // configurations, j > i for the others) in order of distance, so that
+    // consecutive pairs read neighbouring rows of the tabulated network; per
+    // pair (unit vector, d), the cubic Hermite weights on the tabulation
+    // grid (h00, h h10, h01, h h11), and (fc, fc', grid cell, 0) with the
+    // reference's fixed taper (src/nn2b.cu:80-92)
+    const NN2BGrid grid = nn2b_grid_desc();
+    std::vector<float4> pt;
+    pt.reserve(3 * (size_t)nl_.total);
+    std::vector<int4> hdr(std::max(ts_.N, 1));
+    std::vector<int> order;
+    for (int i = 0; i < ts_.N; ++i) {
+      const int nn = nl_.NN[i], b0 = nl_.start[i];
+      const bool all = i < ts_.N_force;
+      order.clear();
+      for (int u = 0; u < nn; ++u)
+        if (all || nl_.NL[b0 + u] > i) order.push_back(b0 + u);
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int a, int b) { return geo_.bond[a].w < geo_.bond[b].w; });
+      hdr[i] = make_int4((int)(pt.size() / 3), (int)order.size(), 0, 0);
+      for (int b : order) {
+        const Vec4 r = geo_.bond[b];
+        const double d = r.w, dinv = 1.0 / d;
+        const double u = (d - (double)grid.d0) / (double)grid.h;
+        const int g = std::max(0, std::min((int)u, grid.G - 2));
+        const double t = u - g, t2 = t * t, t3 = t2 * t, h = grid.h;
+        double fc = 0.0, fcp = 0.0;
+        if (d < NN2B_R1) {
+          fc = 1.0;
+        } else if (d < NN2B_R2) {
+          const double a = (double)NN2B_PI_FACTOR * (d - NN2B_R1);
+          fc = 0.5 * std::cos(a) + 0.5;
+          fcp = -0.5 * (double)NN2B_PI_FACTOR * std::sin(a);
+        }
+        pt.push_back(make_float4((float)(r.x * dinv), (float)(r.y * dinv), (float)(r.z * dinv),
+                                 r.w));
+        pt.push_back(make_float4((float)(2 * t3 - 3 * t2 + 1), (float)(h * (t3 - 2 * t2 + t)),
+                                 (float)(-2 * t3 + 3 * t2), (float)(h * (t3 - t2))));
+        float4 c = make_float4((float)fc, (float)fcp, 0.0f, 0.0f);
+        std::memcpy(&c.z, &g, sizeof g);
+        pt.push_back(c);
+      }
+    }
+    if (pt.empty()) pt.resize(3);
+    ds_.pair_tab.upload(pt, st);
+    ds_.pair_hdr.upload(hdr, st);
   }
   ds_.pe_ref.upload(ts_.pe_ref, st);
   ds_.virial_ref.upload(ts_.virial_ref, st);
@@ -267,6 +352,8 @@ void Fitness::setup()
   v.tri0 = ds_.tri0.get();
   v.tri = ds_.tri.get();
   v.rec = ds_.rec.get();
+  v.pair_hdr = ds_.pair_hdr.get();
+  v.pair_tab = ds_.pair_tab.get();
   v.pe_ref = ds_.pe_ref.get();
   v.virial_ref = ds_.virial_ref.get();
   v.force_ref = ds_.force_ref.get();
@@ -408,22 +495,52 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   }
   std::stable_sort(chunks.begin(), chunks.end(),
                    [](const Chunk& a, const Chunk& b) { return a.cost > b.cost; });
-  std::vector<WorkItem> items;
-  items.reserve(chunks.size() * K * 2);
+  // A last lane block of at most 32 individuals runs packed: G groups of W
+  // lanes, each group on every G-th atom of the chunk.
+  int G_last = 1;
+  if (tersoff && opt_.pack_lanes) {
+    const int Pr = P - (K - 1) * WAVE;
+    int W = 1;
+    while (W < Pr) W <<= 1;
+    G_last = WAVE / W;
+  }
+  pl.groups_last = G_last;
+  struct Costed {
+    WorkItem w;
+    long long cost;
+  };
+  std::vector<Costed> eval;
+  eval.reserve(chunks.size() * K);
   for (const Chunk& ch : chunks)
-    for (int k = 0; k < K; ++k)
-      items.push_back(WorkItem{ch.c, k, ch.chunk, ch.a0, ch.a1, ch.cls, 0, 0});
+    for (int k = 0; k < K; ++k) {
+      const int G = k == K - 1 ? G_last : 1;
+      const int n = ch.a1 - ch.a0, steps = (n + G - 1) / G;
+      eval.push_back(Costed{WorkItem{ch.c, k, ch.chunk, ch.a0, ch.a1, ch.cls, 0, G},
+                            ch.cost * steps / n});
+    }
+  std::stable_sort(eval.begin(), eval.end(),
+                   [](const Costed& a, const Costed& b) { return a.cost > b.cost; });
+  // packed items form a queue of their own, served by a second kernel
+  // instantiation (its per-lane geometry loads need more registers)
+  std::vector<WorkItem> items, packed;
+  items.reserve(chunks.size() * K * 2);
+  for (const Costed& e : eval) (e.w.groups > 1 ? packed : items).push_back(e.w);
   // three-body force configurations: force-assembly items after every
-  // evaluation item (the order the kernel's dependency waits rely on)
+  // evaluation item of their queue (the order the kernel's dependency waits
+  // rely on)
   if (tersoff)
     for (const Chunk& ch : chunks)
       if (ch.c < ts_.Nc_force)
-        for (int k = 0; k < K; ++k)
-          items.push_back(WorkItem{ch.c, k, ch.chunk, ch.a0, ch.a1, ch.cls, 1, 0});
+        for (int k = 0; k < K; ++k) {
+          const int G = k == K - 1 ? G_last : 1;
+          (G > 1 ? packed : items).push_back(WorkItem{ch.c, k, ch.chunk, ch.a0, ch.a1, ch.cls, 1, G});
+        }
   pl.n_items = (int)items.size();
+  pl.n_packed = (int)packed.size();
 
   hipStream_t st = stream_;
   pl.items.upload(items, st);
+  pl.items_packed.upload(packed, st);
   pl.cfg_nchunks.upload(nch, st);
   pl.cfg_slot0.upload(slot0, st);
   pl.part.resize((size_t)std::max(slots, 1) * K * 8 * WAVE);
@@ -438,8 +555,8 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   const int groups = (Nc + CFG_GROUP - 1) / CFG_GROUP;
   pl.gpart.resize((size_t)K * groups * 3 * WAVE);
   // counters: cnt [Nc*K] | cnt2 [Nc*K] | grp_done [K*groups] | blk_done [K] |
-  // fetch | status
-  pl.counters.resize((size_t)2 * Nc * K + (size_t)K * groups + K + 2);
+  // fetch | status | fetch of the packed queue
+  pl.counters.resize((size_t)2 * Nc * K + (size_t)K * groups + K + 3);
   pl.counters.zero(st);
   pl.h_genes.resize((size_t)P * spec_.V);
   pl.h_fitness.resize(P);
@@ -474,18 +591,32 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   c.terms = pl.terms.get();
   c.norm = norm();
 
-  int occ = opt_.blocks_per_cu;
+  PopCtl& cp = pl.ctl_packed;
+  cp = c;
+  cp.items = pl.items_packed.get();
+  cp.n_items = pl.n_packed;
+  cp.fetch = c.status + 1;
+
+  const int cls = nn_class(nl_.max_nn);
+  const bool two = spec_.type == POT_TERSOFF2;
+  int occ = opt_.blocks_per_cu, occ_packed = opt_.blocks_per_cu;
   if (occ <= 0) {
     switch (spec_.type) {
     case POT_RI: occ = ri_population_occupancy(pl.block); break;
     case POT_NN2B: occ = nn2b_population_occupancy(pl.block); break;
-    default: occ = tersoff_population_occupancy(nn_class(nl_.max_nn), spec_.type == POT_TERSOFF2, pl.block); break;
+    default:
+      occ = tersoff_population_occupancy(cls, two, false, pl.block);
+      occ_packed = tersoff_population_occupancy(cls, two, true, pl.block);
+      break;
     }
     occ = std::max(occ, 1);
+    occ_packed = std::max(occ_packed, 1);
   }
   const int waves_per_block = pl.block / WAVE;
   const int need = (pl.n_items + waves_per_block - 1) / waves_per_block;
   pl.grid = std::max(1, std::min(need, num_cu_ * occ));
+  const int need_packed = (pl.n_packed + waves_per_block - 1) / waves_per_block;
+  pl.grid_packed = std::max(1, std::min(need_packed, num_cu_ * occ_packed));
   GPUGA_HIP(hipStreamSynchronize(st));
 }
 
@@ -519,9 +650,25 @@ void Fitness::launch_population(PopPlan& pl, hipStream_t st)
     launch_nn2b_population(pl.ctl, ds_.view, g, pl.nn2b_table.get(), pl.grid, pl.block, st);
     break;
   }
-  default:
-    launch_tersoff_population(pl.ctl, ds_.view, nn_class(nl_.max_nn), pl.grid, pl.block, st);
+  default: {
+    // the packed queue runs on a second stream, forked from and joined back
+    // into the launch stream (two parallel nodes when captured), so its
+    // waves fill the slots the main launch frees as it drains
+    const int cls = nn_class(nl_.max_nn);
+    if (pl.n_packed > 0) {
+      GPUGA_HIP(hipEventRecord(ev_fork_, st));
+      GPUGA_HIP(hipStreamWaitEvent(side_, ev_fork_, 0));
+    }
+    if (pl.n_items > 0)
+      launch_tersoff_population(pl.ctl, ds_.view, cls, pl.grid, pl.block, false, st);
+    if (pl.n_packed > 0) {
+      launch_tersoff_population(pl.ctl_packed, ds_.view, cls, pl.grid_packed, pl.block, true,
+                                side_);
+      GPUGA_HIP(hipEventRecord(ev_join_, side_));
+      GPUGA_HIP(hipStreamWaitEvent(st, ev_join_, 0));
+    }
     break;
+  }
   }
 }
 
@@ -692,7 +839,8 @@ size_t Fitness::estimate_bytes(const PotentialSpec& s, const TrainSet& ts, int P
   size_t tri = 0;
   for (int i = 0; i < ts.N; ++i) tri += (size_t)nl.NN[i] * nl.NN[i];
   size_t b = N * sizeof(int) * 5 + Nc * sizeof(int) * 2 + B * (3 * sizeof(int) + 16);
-  b += (tersoff ? tri * 16 + B * 4 * 5 : 0) + (s.type == POT_RI ? B * 16 : 0);
+  b += (tersoff ? tri * 16 + B * 4 * 5 : 0) + (s.type == POT_RI ? B * 48 + N * 16 : 0);
+  b += s.type == POT_NN2B ? B * 48 + N * 16 : 0;
   b += Nc * 7 * sizeof(float) + N * 3 * sizeof(float);  // references
   b += N * 10 * sizeof(float) + Nc * 3 * sizeof(double); // per-atom outputs
   const size_t K = (P + WAVE - 1) / WAVE;
@@ -708,19 +856,24 @@ std::vector<double> Fitness::trace(int P, const float* genes)
 {
   PopPlan& pl = plan(P);
   hipStream_t st = stream_;
-  const size_t n = pl.n_items;
-  DeviceBuffer<unsigned long long> buf(4 * n);
+  // main queue, then the packed queue (whose wave slots follow the main ones)
+  const size_t n0 = pl.n_items, n = n0 + pl.n_packed;
+  DeviceBuffer<unsigned long long> buf(std::max<size_t>(4 * n, 1));
   buf.zero(st);
   std::vector<WorkItem> items(n);
-  pl.items.download(items.data(), n, st);
+  pl.items.download(items.data(), n0, st);
+  pl.items_packed.download(items.data() + n0, pl.n_packed, st);
   GPUGA_HIP(hipMemcpyAsync(pl.genes.get(), genes, sizeof(float) * (size_t)P * spec_.V,
                            hipMemcpyHostToDevice, st));
   pl.ctl.trace = buf.get();
+  pl.ctl_packed.trace = buf.get() + 4 * n0;
   launch_population(pl, st);
   pl.ctl.trace = nullptr;
+  pl.ctl_packed.trace = nullptr;
   std::vector<unsigned long long> t(4 * n);
   buf.download(t.data(), t.size(), st);
   GPUGA_HIP(hipStreamSynchronize(st));
+  for (size_t i = n0; i < n; ++i) t[4 * i] += (unsigned long long)pl.grid * (pl.block / WAVE);
   int khz = 0;
   GPUGA_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_));
   const double ns_per_tick = 1e6 / std::max(khz, 1);

@@ -21,12 +21,14 @@ struct FitnessOptions {
   int nn2b_grid = 1024;   // NN2B tabulation points
   int verbose = 0;
   int blocks_per_cu = 0;  // 0 = from the occupancy calculator
+  int pack_lanes = 1;     // pack a last lane block of <= 32 individuals
 };
 
 // Device-resident copy of one training set and its geometry tables.
 struct DeviceSet {
   DeviceBuffer<int> Na, Na_sum, type, nn, bond0, nl, rev, ptype, tri0;
-  DeviceBuffer<float4> bond, tri, ri_tab, rec;
+  DeviceBuffer<float4> bond, tri, ri_tab, rec, pair_tab;
+  DeviceBuffer<int4> pair_hdr;
   DeviceBuffer<float> pe_ref, virial_ref, force_ref;
   DevSet view{};
   size_t bytes() const;
@@ -49,7 +51,9 @@ std::unique_ptr<Potential> make_potential(const PotentialSpec& spec, const Devic
 // One population plan (work items, scratch) for a population size P.
 struct PopPlan {
   int P = 0, K = 0, n_items = 0, grid = 0, block = 256, chunk_cost = 0;
-  DeviceBuffer<WorkItem> items;
+  int groups_last = 1; // lane groups of the last lane block's items
+  int n_packed = 0, grid_packed = 0; // packed-item queue (its own kernel)
+  DeviceBuffer<WorkItem> items, items_packed;
   DeviceBuffer<int> cfg_nchunks, cfg_slot0;
   DeviceBuffer<double> part, err, gpart, terms;
   DeviceBuffer<float> bondf, genes, fitness, nn2b_table;
@@ -57,7 +61,7 @@ struct PopPlan {
   PinnedBuffer<float> h_genes, h_fitness;
   PinnedBuffer<double> h_terms;
   PinnedBuffer<unsigned> h_status;
-  PopCtl ctl{};
+  PopCtl ctl{}, ctl_packed{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~PopPlan();
@@ -134,6 +138,8 @@ private:
   DeviceBuffer<float> d_f_, d_virial_, d_pe_, d_params_, d_fit1_, d_pmin_, d_pmax_;
   DeviceBuffer<double> d_cerr_, d_terms1_;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  hipStream_t side_ = nullptr; // second stream of the packed-item launch
+  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
   bool timing_ = false;
   double kernel_ms_ = 0.0;
   long kernel_launches_ = 0;

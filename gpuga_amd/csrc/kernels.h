@@ -24,9 +24,10 @@ void launch_fitness_terms(const DevSet& s, const FitnessNorm& nrm, const float* 
                           double* terms, hipStream_t st);
 
 // ---- population kernels (one persistent launch per generation)
-// cls: neighbour-count class of the training set (0: <= 4, 1: <= 8, 2: <= 16)
+// cls: neighbour-count class of the training set (0: <= 4, 1: <= 8, 2: <= 16);
+// pack: serve the packed-item queue (several atoms per wave)
 void launch_tersoff_population(const PopCtl& ctl, const DevSet& s, int cls, int grid, int block,
-                               hipStream_t st);
+                               bool pack, hipStream_t st);
 void launch_ri_population(const PopCtl& ctl, const DevSet& s, const float4* ri_tab, int grid,
                           int block, hipStream_t st);
 

@@ -83,6 +83,7 @@ __global__ void __launch_bounds__(256) nn2b_tabulate_f32(PopCtl ctl, NN2BGrid gr
 
 struct NN2BPop {
   static constexpr bool kForcePass = false;
+  static constexpr bool kPacked = false;
   struct Lane {
     int dummy;
   };
@@ -93,40 +94,21 @@ struct NN2BPop {
 
   __device__ static Lane make(const PopCtl&, int) { return Lane{0}; }
 
-  // p(r) and p'(r) of this lane's individual by cubic Hermite interpolation
-  __device__ static void eval(const Extra& ex, const float* __restrict__ tk, float d, float& p,
-                              float& pd)
+  // One pair from its table entry u = (unit vector, d), w = Hermite weights,
+  // c = (fc, fc', grid cell): p = U fc and G = d f2 = U' fc + (U fc) fc'
+  // (the reference's force expression, src/nn2b.cu:143-148), with U and U'
+  // of this lane's individual interpolated from the rows (p, p', p'') of
+  // the cell's two grid points.
+  __device__ static __forceinline__ void pair(const float* __restrict__ tk, const float4& w,
+                                              const float4& c, float& p2, float& G)
   {
-    float u = (d - ex.g.d0) / ex.g.h;
-    int g = (int)u;
-    g = max(0, min(g, ex.g.G - 2));
-    const float t = u - (float)g;
-    const float t2 = t * t, t3 = t2 * t;
-    const float h00 = 2.0f * t3 - 3.0f * t2 + 1.0f, h10 = t3 - 2.0f * t2 + t;
-    const float h01 = -2.0f * t3 + 3.0f * t2, h11 = t3 - t2;
-    const float* r0 = tk + (size_t)g * 3 * WAVE;
-    const float* r1 = r0 + 3 * WAVE;
+    const float* r0 = tk + (size_t)__float_as_int(c.z) * 3 * WAVE;
     const float p0 = r0[0], d0 = r0[WAVE], s0 = r0[2 * WAVE];
-    const float p1 = r1[0], d1 = r1[WAVE], s1 = r1[2 * WAVE];
-    const float h = ex.g.h;
-    p = h00 * p0 + h10 * h * d0 + h01 * p1 + h11 * h * d1;
-    pd = h00 * d0 + h10 * h * s0 + h01 * d1 + h11 * h * s1;
-  }
-
-  __device__ static void taper(float d, float& fc, float& fcp)
-  {
-    if (d < NN2B_R1) {
-      fc = 1.0f;
-      fcp = 0.0f;
-    } else if (d < NN2B_R2) {
-      float sn, cs;
-      sincosf(NN2B_PI_FACTOR * (d - NN2B_R1), &sn, &cs);
-      fc = 0.5f * cs + 0.5f;
-      fcp = -sn * NN2B_PI_FACTOR * 0.5f;
-    } else {
-      fc = 0.0f;
-      fcp = 0.0f;
-    }
+    const float p1 = r0[3 * WAVE], d1 = r0[4 * WAVE], s1 = r0[5 * WAVE];
+    const float p = fmaf(w.x, p0, fmaf(w.y, d0, fmaf(w.z, p1, w.w * d1)));
+    const float pd = fmaf(w.x, d0, fmaf(w.y, s0, fmaf(w.z, d1, w.w * s1)));
+    p2 = p * c.x;
+    G = fmaf(pd, c.x, p2 * c.y);
   }
 
   __device__ static void chunk(const DevSet& s, const Lane&, const WorkItem& it, const PopCtl&,
@@ -134,47 +116,43 @@ struct NN2BPop {
   {
     const int lane = __lane_id();
     const float* tk = ex.table + (size_t)it.k * ex.g.G * 3 * WAVE + lane;
-    const size_t N = s.N;
+    const long N = s.N;
     if (it.c < s.Nc_force) {
       for (int i = it.a0; i < it.a1; ++i) {
-        const int nn = s.nn[i], b0 = s.bond0[i];
+        const int4 h = ldu(s.pair_hdr, i);
+        const float4* tab = s.pair_tab + 3 * (long)h.x;
         float fx = 0, fy = 0, fz = 0;
-        for (int u = 0; u < nn; ++u) {
-          const float4 r = s.bond[b0 + u];
-          float p, pd, fc, fcp;
-          eval(ex, tk, r.w, p, pd);
-          taper(r.w, fc, fcp);
-          const float p2 = p * fc;
-          const float f2 = (pd * fc + p2 * fcp) / r.w;
-          fx += r.x * f2;
-          fy += r.y * f2;
-          fz += r.z * f2;
+        for (int k = 0; k < h.y; ++k) {
+          const float4 u = ldu(tab, 3 * k), w = ldu(tab, 3 * k + 1), c = ldu(tab, 3 * k + 2);
+          float p2, G;
+          pair(tk, w, c, p2, G);
+          fx = fmaf(u.x, G, fx);
+          fy = fmaf(u.y, G, fy);
+          fz = fmaf(u.z, G, fz);
         }
-        const float dx = fx - s.force_ref[i], dy = fy - s.force_ref[i + N],
-                    dz = fz - s.force_ref[i + 2 * N];
+        const float dx = fx - ldu(s.force_ref, i), dy = fy - ldu(s.force_ref, i + N),
+                    dz = fz - ldu(s.force_ref, i + 2 * N);
         acc.v[7] += (double)dx * dx + (double)dy * dy + (double)dz * dz;
       }
     } else {
+      // each unordered pair once (the table holds j > i), full weight
       for (int i = it.a0; i < it.a1; ++i) {
-        const int nn = s.nn[i], b0 = s.bond0[i];
+        const int4 h = ldu(s.pair_hdr, i);
+        const float4* tab = s.pair_tab + 3 * (long)h.x;
         float e = 0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wyz = 0, wzx = 0;
-        for (int u = 0; u < nn; ++u) {
-          const int b = b0 + u;
-          if (s.nl[b] < i) continue; // each unordered pair once, full weight
-          const float4 r = s.bond[b];
-          float p, pd, fc, fcp;
-          eval(ex, tk, r.w, p, pd);
-          taper(r.w, fc, fcp);
-          const float p2 = p * fc;
-          const float f2 = (pd * fc + p2 * fcp) / r.w;
-          const float rx = r.x * f2, ry = r.y * f2, rz = r.z * f2;
-          wxx -= r.x * rx;
-          wyy -= r.y * ry;
-          wzz -= r.z * rz;
-          wxy -= r.x * ry;
-          wyz -= r.y * rz;
-          wzx -= r.z * rx;
+        for (int k = 0; k < h.y; ++k) {
+          const float4 u = ldu(tab, 3 * k), w = ldu(tab, 3 * k + 1), c = ldu(tab, 3 * k + 2);
+          float p2, G;
+          pair(tk, w, c, p2, G);
+          const float t = u.w * G;
+          const float tx = u.x * t, ty = u.y * t, tz = u.z * t;
           e += p2;
+          wxx = fmaf(-u.x, tx, wxx);
+          wyy = fmaf(-u.y, ty, wyy);
+          wzz = fmaf(-u.z, tz, wzz);
+          wxy = fmaf(-u.x, ty, wxy);
+          wyz = fmaf(-u.y, tz, wyz);
+          wzx = fmaf(-u.z, tx, wzx);
         }
         acc.v[0] += e;
         acc.v[1] += wxx;

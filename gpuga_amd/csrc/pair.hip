@@ -78,9 +78,67 @@ __global__ void ri_plain(DevSet s, RIParams q, const float4* __restrict__ tab,
   pe[i] = e;
 }
 
+// Per-individual RI constants in the population form: exp(-d/rho) as
+// exp2(-d b2) with b2 = log2(e)/rho, and the shifted Coulomb terms
+// qq (erfc(a d)/d - v_rc - dv_rc (d - rc)) as qq erfc(a d)/d - k1 d - k0.
+struct RILane {
+  float a[3], b[3], b2[3], c[3], qq[3], k0[3], k1[3];
+  float mR, kR, kR2, mA, kA, kA2, r0;
+  int morse; // wave-uniform: some lane of the wave has a Morse term
+};
+
+__device__ inline RILane ri_lane(const RIParams& p)
+{
+  RILane q;
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    q.a[t] = p.a[t];
+    q.b[t] = p.b[t];
+    q.b2[t] = p.b[t] * LOG2E;
+    q.c[t] = p.c[t];
+    q.qq[t] = p.qq[t];
+    q.k0[t] = p.qq[t] * (p.v_rc - p.dv_rc * p.cutoff);
+    q.k1[t] = p.qq[t] * p.dv_rc;
+  }
+  q.mR = p.mR;
+  q.kR = p.kR;
+  q.kR2 = p.kR * LOG2E;
+  q.mA = p.mA;
+  q.kA = p.kA;
+  q.kA2 = p.kA * LOG2E;
+  q.r0 = p.r0;
+  q.morse = __any(p.mR != 0.0f || p.mA != 0.0f);
+  return q;
+}
+
+// One pair of type T (src/ri.cu:77-110) from its table entry
+// u = (unit vector, d), f = (d^-6, 6 d^-7, erfc(a d)/d, -(d/dr)(erfc(a d)/d)):
+// g = U'(d) and, when ENERGY, p = U(d).
+template <int T, bool ENERGY>
+__device__ __forceinline__ void ri_pair_pop(const RILane& q, float d, const float4& f, float& p,
+                                            float& g)
+{
+  const float aex = q.a[T] * fexp2(-d * q.b2[T]);
+  g = fmaf(q.c[T], f.y, -fmaf(aex, q.b[T], q.k1[T]));
+  g = fmaf(-q.qq[T], f.w, g);
+  if (ENERGY) {
+    p = fmaf(q.qq[T], f.z, aex);
+    p = fmaf(-q.c[T], f.x, p);
+    p = fmaf(-q.k1[T], d, p) - q.k0[T];
+  }
+  if (T == 1 && q.morse) {
+    const float x = d - q.r0;
+    const float fr = q.mR * fexp2(q.kR2 * x);
+    const float fa = q.mA * fexp2(q.kA2 * x);
+    if (ENERGY) p += fr - fa;
+    g += q.kR * fr - q.kA * fa;
+  }
+}
+
 struct RIPop {
   static constexpr bool kForcePass = false;
-  using Lane = RIParams;
+  static constexpr bool kPacked = false;
+  using Lane = RILane;
   using Extra = const float4*;
 
   __device__ static Lane make(const PopCtl& ctl, int p)
@@ -88,60 +146,78 @@ struct RIPop {
     float x[15];
 #pragma unroll
     for (int v = 0; v < 15; ++v)
-      x[v] = denormalise(ctl.pmin[v], ctl.pmax[v], ctl.genes[(size_t)p * ctl.V + v]);
-    return ri_params(x);
+      x[v] = denormalise(ldu(ctl.pmin, v), ldu(ctl.pmax, v), ctl.genes[(size_t)p * ctl.V + v]);
+    return ri_lane(ri_params(x));
   }
 
-  // Energy/virial configurations visit each unordered pair once (bonds with
-  // j > i, whose reverse carries the same |r| and -r); force configurations
-  // visit every bond of an atom so that F_i is complete in the lane.
-  __device__ static void chunk(const DevSet& s, const RIParams& q, const WorkItem& it,
-                               const PopCtl&, const float4* __restrict__ tab, Acc& acc)
+  // The pairs of type T of one atom: F_i += u g (force configurations) or
+  // E += U, W -= u (x) u d g over the unordered pairs (the others).  The
+  // geometry is wave-uniform (scalar loads); two pairs per step.
+  template <int T, bool FORCE>
+  __device__ static void pairs(const RILane& q, const float4* tab, int n, float* a)
   {
-    const size_t N = s.N;
+    int k = 0;
+    for (; k + 2 <= n; k += 2) {
+      const float4 u0 = ldu(tab, 2 * k), f0 = ldu(tab, 2 * k + 1);
+      const float4 u1 = ldu(tab, 2 * k + 2), f1 = ldu(tab, 2 * k + 3);
+      one<T, FORCE>(q, u0, f0, a);
+      one<T, FORCE>(q, u1, f1, a);
+    }
+    if (k < n) one<T, FORCE>(q, ldu(tab, 2 * k), ldu(tab, 2 * k + 1), a);
+  }
+
+  template <int T, bool FORCE>
+  __device__ static __forceinline__ void one(const RILane& q, const float4& u, const float4& f,
+                                             float* a)
+  {
+    float p, g;
+    ri_pair_pop<T, !FORCE>(q, u.w, f, p, g);
+    if (FORCE) {
+      a[0] = fmaf(u.x, g, a[0]);
+      a[1] = fmaf(u.y, g, a[1]);
+      a[2] = fmaf(u.z, g, a[2]);
+    } else {
+      const float t = u.w * g;
+      const float tx = u.x * t, ty = u.y * t, tz = u.z * t;
+      a[0] += p;
+      a[1] = fmaf(-u.x, tx, a[1]);
+      a[2] = fmaf(-u.y, ty, a[2]);
+      a[3] = fmaf(-u.z, tz, a[3]);
+      a[4] = fmaf(-u.x, ty, a[4]);
+      a[5] = fmaf(-u.y, tz, a[5]);
+      a[6] = fmaf(-u.z, tx, a[6]);
+    }
+  }
+
+  // Energy/virial configurations visit each unordered pair once (the table
+  // holds only j > i for their atoms: the reverse bond carries the same d and
+  // -r); force configurations visit every bond so F_i is complete in the lane.
+  __device__ static void chunk(const DevSet& s, const RILane& q, const WorkItem& it,
+                               const PopCtl&, const float4*, Acc& acc)
+  {
+    const long N = s.N;
     if (it.c < s.Nc_force) {
       for (int i = it.a0; i < it.a1; ++i) {
-        const int nn = s.nn[i], b0 = s.bond0[i];
-        float fx = 0, fy = 0, fz = 0;
-        for (int u = 0; u < nn; ++u) {
-          const int b = b0 + u;
-          const float4 r = s.bond[b];
-          float p2, f2;
-          ri_pair(q, s.ptype[b], r.w, tab[b], p2, f2);
-          fx += r.x * f2;
-          fy += r.y * f2;
-          fz += r.z * f2;
-        }
-        const float dx = fx - s.force_ref[i], dy = fy - s.force_ref[i + N],
-                    dz = fz - s.force_ref[i + 2 * N];
+        const int4 h = ldu(s.pair_hdr, i);
+        const float4* tab = s.pair_tab + 2 * (long)h.x;
+        float a[3] = {0.0f, 0.0f, 0.0f};
+        pairs<0, true>(q, tab, h.y, a);
+        pairs<1, true>(q, tab + 2 * (long)h.y, h.z, a);
+        pairs<2, true>(q, tab + 2 * (long)(h.y + h.z), h.w, a);
+        const float dx = a[0] - ldu(s.force_ref, i), dy = a[1] - ldu(s.force_ref, i + N),
+                    dz = a[2] - ldu(s.force_ref, i + 2 * N);
         acc.v[7] += (double)dx * dx + (double)dy * dy + (double)dz * dz;
       }
     } else {
       for (int i = it.a0; i < it.a1; ++i) {
-        const int nn = s.nn[i], b0 = s.bond0[i];
-        float e = 0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wyz = 0, wzx = 0;
-        for (int u = 0; u < nn; ++u) {
-          const int b = b0 + u;
-          if (s.nl[b] < i) continue;
-          const float4 r = s.bond[b];
-          float p2, f2;
-          ri_pair(q, s.ptype[b], r.w, tab[b], p2, f2);
-          const float rx = r.x * f2, ry = r.y * f2, rz = r.z * f2;
-          wxx -= r.x * rx;
-          wyy -= r.y * ry;
-          wzz -= r.z * rz;
-          wxy -= r.x * ry;
-          wyz -= r.y * rz;
-          wzx -= r.z * rx;
-          e += p2;
-        }
-        acc.v[0] += e;
-        acc.v[1] += wxx;
-        acc.v[2] += wyy;
-        acc.v[3] += wzz;
-        acc.v[4] += wxy;
-        acc.v[5] += wyz;
-        acc.v[6] += wzx;
+        const int4 h = ldu(s.pair_hdr, i);
+        const float4* tab = s.pair_tab + 2 * (long)h.x;
+        float a[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        pairs<0, false>(q, tab, h.y, a);
+        pairs<1, false>(q, tab + 2 * (long)h.y, h.z, a);
+        pairs<2, false>(q, tab + 2 * (long)(h.y + h.z), h.w, a);
+#pragma unroll
+        for (int u = 0; u < 7; ++u) acc.v[u] += (double)a[u];
       }
     }
   }

@@ -20,6 +20,15 @@
 // such wait ends; it is bounded anyway, and a timeout is reported through a
 // status word instead of hanging.
 //
+// Packed lane block.  When the last lane block holds at most 32 individuals
+// (P mod 64), its items split the wave into G = 64 / W groups of W lanes
+// (W the power of two >= that count): lane l evaluates individual 64k + l % W
+// on the atoms a0 + l / W, a0 + l / W + G, ... of the chunk, with per-lane
+// geometry loads, and the G group partials are summed by a fixed butterfly
+// before the chunk result is stored.  Afterwards lane column l holds the
+// result of individual 64k + l % W, so every later step is unchanged (the
+// columns past the block's individuals are never read).
+//
 // Reductions.  The last chunk of a (configuration, lane block) folds the chunk
 // partials in chunk order and writes the configuration's error terms; the last
 // configuration of a group of 64 sums the group in index order; the last group
@@ -46,7 +55,7 @@ struct WorkItem {
   int a0, a1; // global atom range of the chunk
   int cls;    // neighbour-count class of the chunk
   int phase;  // 0: evaluate, 1: assemble forces of a three-body force configuration
-  int pad;
+  int groups; // 1: lane = individual; G > 1: packed lane block (see below)
 };
 
 struct PopCtl {
@@ -222,6 +231,14 @@ __device__ inline bool fold_chunks(const PopCtl& ctl, unsigned* counter, int c, 
   return true;
 }
 
+// Sum over the G lane groups of a packed item: lanes l, l ^ W, l ^ 2W, ...
+// (fixed order; every lane ends with its column's total).
+__device__ inline double group_sum(double x, int W)
+{
+  for (int m = W; m < WAVE; m <<= 1) x += __shfl_xor(x, m);
+  return x;
+}
+
 // Pot must provide:
 //   struct Lane; typedef Extra;                     per-lane constants, tables
 //   static Lane make(ctl, p);                       denormalised genes of p
@@ -229,7 +246,14 @@ __device__ inline bool fold_chunks(const PopCtl& ctl, unsigned* counter, int c, 
 //   static constexpr bool kForcePass;               three-body force assembly
 //   static double force_pass(set, ctl, a0, a1, k, lane)
 //                                                   sum |F-F_ref|^2 over [a0, a1)
-template <class Pot>
+//   static constexpr bool kPacked;                  packed items supported:
+//   static void chunk_packed(set, q, item, ctl, ex, col, g, G, Acc&)
+//   static double force_pass_packed(set, ctl, a0, a1, k, col, g, G)
+//                                                   the same for lane group g
+// PACK selects the instantiation that serves the packed-item queue; the
+// ordinary one never contains the packed code, so its register budget is
+// unaffected.
+template <class Pot, bool PACK = false>
 __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
                                                          typename Pot::Extra ex)
 {
@@ -263,11 +287,23 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc.v[u] = 0.0;
 
+    constexpr bool packed = PACK && Pot::kPacked;
     if (item.phase == 0) {
-      // individual of this lane; lanes past P repeat the last one, write nothing
-      const int p = min(k * WAVE + lane, ctl.P - 1);
-      const typename Pot::Lane q = Pot::make(ctl, p);
-      Pot::chunk(set, q, item, ctl, ex, acc);
+      if constexpr (packed) {
+        const int G = item.groups, W = WAVE / G;
+        const int col = lane & (W - 1);
+        const typename Pot::Lane q = Pot::make(ctl, min(k * WAVE + col, ctl.P - 1));
+        Pot::chunk_packed(set, q, item, ctl, ex, col, lane / W, G, acc);
+        if (!(Pot::kForcePass && is_force)) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc.v[u] = group_sum(acc.v[u], W);
+        }
+      } else {
+        // individual of this lane; lanes past P repeat the last one, write nothing
+        const int p = min(k * WAVE + lane, ctl.P - 1);
+        const typename Pot::Lane q = Pot::make(ctl, p);
+        Pot::chunk(set, q, item, ctl, ex, acc);
+      }
       if (Pot::kForcePass && is_force) {
         publish(); // the per-bond forces of this chunk are complete
         wave_atomic_add(cnt, 1u);
@@ -281,7 +317,13 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
       if (!wave_wait_count(cnt, (unsigned)nch)) {
         if (lane == 0) atomicOr(ctl.status, 1u);
       }
-      acc.v[7] = Pot::force_pass(set, ctl, item.a0, item.a1, k, lane);
+      if constexpr (packed) {
+        const int G = item.groups, W = WAVE / G;
+        acc.v[7] = group_sum(
+          Pot::force_pass_packed(set, ctl, item.a0, item.a1, k, lane & (W - 1), lane / W, G), W);
+      } else {
+        acc.v[7] = Pot::force_pass(set, ctl, item.a0, item.a1, k, lane);
+      }
       unsigned* cnt2 = &ctl.cnt2[(long)c * ctl.K + k];
       if (nch > 1) {
         if (!fold_chunks(ctl, cnt2, c, k, item.chunk, nch, lane, acc)) continue;

@@ -230,22 +230,35 @@ struct TersoffPop {
 
   // One atom for 64 individuals: the geometry is wave-uniform (scalar loads
   // from the atom's record), the parameters are per lane; all loop guards are
-  // scalar branches.
-  // record stride in float4: header, bonds, triplets, reverse-bond indices
-  static constexpr int RS = 1 + NMAX + NMAX * NMAX + NMAX / 4;
+  // scalar branches (U = false: per-lane atoms of a packed item, ordinary
+  // loads).
+  // record stride in float4: header, bonds, triplets, reverse-bond indices,
+  // inverse distances
+  static constexpr int RS = 1 + NMAX + NMAX * NMAX + NMAX / 2;
+  static constexpr int REV = 1 + NMAX + NMAX * NMAX; // reverse-bond indices
+  static constexpr int INV = REV + NMAX / 4;          // inverse distances
 
-  template <bool FORCE>
+  template <bool FORCE, bool U>
   __device__ static void atom(const DevSet& s, const TLane& q, int i, float* fblk, int lane,
                               Acc& acc)
   {
     const float4* R = s.rec + (long)i * RS;
-    const int4 hd = ldu(reinterpret_cast<const int4*>(R), 0);
+    // Every load is unconditional (records are zero-padded to the bound), so
+    // the loads of a row issue together and are waited for once, instead of
+    // one load and wait per guarded use.
+    const int4 hd = ldx<U>(reinterpret_cast<const int4*>(R), 0);
+    float d[NMAX], dinv[NMAX];
+#pragma unroll
+    for (int u = 0; u < NMAX; ++u) {
+      d[u] = ldx<U>(R, 1 + u).w;
+      dinv[u] = ldx<U>(reinterpret_cast<const float*>(R + INV), u);
+    }
     const int nn = hd.x, unl = hd.y;
     float fc[NMAX], fcp[NMAX], fcb[NMAX], A[NMAX], B[NMAX], pf[NMAX];
 #pragma unroll
     for (int u = 0; u < NMAX; ++u) {
       if (u < nn) {
-        tersoff_fc(q, ldu(R, 1 + u).w, fc[u], fcp[u]);
+        tersoff_fc(q, d[u], fc[u], fcp[u]);
         fcb[u] = fc[u] * q.beta;
       }
     }
@@ -253,25 +266,27 @@ struct TersoffPop {
 #pragma unroll
     for (int u = 0; u < NMAX; ++u) {
       if (u < nn) {
+        float cs[NMAX];
+#pragma unroll
+        for (int v = 0; v < NMAX; ++v)
+          if (v != u) cs[v] = ldx<U>(R, 1 + NMAX + u * NMAX + v).x;
         float zeta = 0.0f;
 #pragma unroll
         for (int v = 0; v < NMAX; ++v) {
           if (v < nn && v != u) {
-            const float x = ldu(R, 1 + NMAX + u * NMAX + v).x - q.h;
+            const float x = cs[v] - q.h;
             zeta = fmaf(fcb[v], x * x, zeta);
           }
         }
         float b, bp;
         tersoff_bond_order(q, zeta, b, bp);
-        const float d = ldu(R, 1 + u).w;
         const int k = TWO ? ((unl >> u) & 1) : 0;
         float fr, frp, fa, fap;
-        tersoff_pair(q, k, d, fr, frp, fa, fap);
+        tersoff_pair(q, k, d[u], fr, frp, fa, fap);
         const float rep = fr - b * fa;
-        const float dinv = frcp(d);
         e = fmaf(fc[u], rep, e);
-        pf[u] = (fcp[u] * rep + fc[u] * (frp - b * fap)) * dinv;
-        fcp[u] *= dinv;
+        pf[u] = (fcp[u] * rep + fc[u] * (frp - b * fap)) * dinv[u];
+        fcp[u] *= dinv[u];
         A[u] = bp * fa;
         B[u] = A[u] * fc[u];
       }
@@ -280,29 +295,37 @@ struct TersoffPop {
 #pragma unroll
     for (int u = 0; u < NMAX; ++u) {
       if (u < nn) {
+        // three-body part of f_ij with beta factored out of both sums:
+        //   c    = sum_k (h - cos)(A_j + A_k) fc_k dcos/dr   (times 2 beta)
+        //   dsum = sum_k B_k (h - cos)^2                     (times beta)
+        // where (A_j + A_k) fc_k = A_j fc_k + B_k
+        float4 row[NMAX];
+#pragma unroll
+        for (int v = 0; v < NMAX; ++v)
+          if (v != u) row[v] = ldx<U>(R, 1 + NMAX + u * NMAX + v);
         float cx = 0.0f, cy = 0.0f, cz = 0.0f, dsum = 0.0f;
 #pragma unroll
         for (int v = 0; v < NMAX; ++v) {
           if (v < nn && v != u) {
-            const float4 t = ldu(R, 1 + NMAX + u * NMAX + v);
+            const float4 t = row[v];
             const float x = t.x - q.h;
-            const float w = fc[v] * (A[u] + A[v]) * (q.beta2 * x);
+            const float w = fmaf(A[u], fc[v], B[v]) * x;
             cx = fmaf(w, t.y, cx);
             cy = fmaf(w, t.z, cy);
             cz = fmaf(w, t.w, cz);
-            dsum = fmaf(B[v], q.beta * x * x, dsum);
+            dsum = fmaf(B[v] * x, x, dsum);
           }
         }
-        const float4 r = ldu(R, 1 + u);
-        const float coef = 0.5f * (pf[u] - fcp[u] * dsum);
-        const float hc = 0.5f * fc[u];
+        const float4 r = ldx<U>(R, 1 + u);
+        const float coef = 0.5f * (pf[u] - fcp[u] * (q.beta * dsum));
+        const float hc = fc[u] * q.beta;
         const float fx = coef * r.x - hc * cx;
         const float fy = coef * r.y - hc * cy;
         const float fz = coef * r.z - hc * cz;
         if (FORCE) {
           // f_ij goes to the slot of the reverse bond, where atom j reads it
           // back as f_ji; F_i's own part sum_j f_ij stays in registers
-          const int rb = ldu(reinterpret_cast<const int*>(R + 1 + NMAX + NMAX * NMAX), u);
+          const int rb = ldx<U>(reinterpret_cast<const int*>(R + REV), u);
           float* o = fblk + (size_t)rb * 3 * WAVE + lane;
           cstore(o, fx);
           cstore(o + WAVE, fy);
@@ -345,9 +368,23 @@ struct TersoffPop {
     const int lane = __lane_id();
     if (it.c < s.Nc_force) {
       float* fblk = ctl.bondf + (size_t)it.k * ctl.B_force * 3 * WAVE;
-      for (int i = it.a0; i < it.a1; ++i) atom<true>(s, q, i, fblk, lane, acc);
+      for (int i = it.a0; i < it.a1; ++i) atom<true, true>(s, q, i, fblk, lane, acc);
     } else {
-      for (int i = it.a0; i < it.a1; ++i) atom<false>(s, q, i, nullptr, lane, acc);
+      for (int i = it.a0; i < it.a1; ++i) atom<false, true>(s, q, i, nullptr, lane, acc);
+    }
+  }
+
+  // Packed item: lane group g takes the atoms a0 + g, a0 + g + G, ...; the
+  // per-bond forces go to lane column col (the lane's individual).
+  static constexpr bool kPacked = true;
+  __device__ static void chunk_packed(const DevSet& s, const TLane& q, const WorkItem& it,
+                                      const PopCtl& ctl, int, int col, int g, int G, Acc& acc)
+  {
+    if (it.c < s.Nc_force) {
+      float* fblk = ctl.bondf + (size_t)it.k * ctl.B_force * 3 * WAVE;
+      for (int i = it.a0 + g; i < it.a1; i += G) atom<true, false>(s, q, i, fblk, col, acc);
+    } else {
+      for (int i = it.a0 + g; i < it.a1; i += G) atom<false, false>(s, q, i, nullptr, col, acc);
     }
   }
 
@@ -392,48 +429,81 @@ struct TersoffPop {
     }
     return err;
   }
+
+  // The same for lane group g of a packed item (atoms a0 + g, a0 + g + G, ...
+  // of lane column col).
+  __device__ static double force_pass_packed(const DevSet& s, const PopCtl& ctl, int a0, int a1,
+                                             int k, int col, int g, int G)
+  {
+    const float* fblk = ctl.bondf + (size_t)k * ctl.B_force * 3 * WAVE + col;
+    const long N = s.N;
+    double err = 0.0;
+    for (int i = a0 + g; i < a1; i += G) {
+      const int4 hd = reinterpret_cast<const int4*>(s.rec)[(long)i * RS];
+      const float* own = fblk + ((size_t)s.bond_force + i) * 3 * WAVE;
+      float fx = cload(own), fy = cload(own + WAVE), fz = cload(own + 2 * WAVE);
+#pragma unroll
+      for (int u = 0; u < NMAX; ++u) {
+        if (u < hd.x) {
+          const float* r = fblk + (size_t)(hd.z + u) * 3 * WAVE;
+          fx -= cload(r);
+          fy -= cload(r + WAVE);
+          fz -= cload(r + 2 * WAVE);
+        }
+      }
+      const float dx = fx - s.force_ref[i], dy = fy - s.force_ref[i + N],
+                  dz = fz - s.force_ref[i + 2 * N];
+      err += (double)dx * dx + (double)dy * dy + (double)dz * dz;
+    }
+    return err;
+  }
 };
 
 template <int NMAX, bool TWO>
-static void launch_t(const PopCtl& ctl, const DevSet& s, int grid, int block, hipStream_t st)
+static void launch_t(const PopCtl& ctl, const DevSet& s, int grid, int block, bool pack,
+                     hipStream_t st)
 {
-  population_kernel<TersoffPop<NMAX, TWO>><<<grid, block, 0, st>>>(ctl, s, 0);
+  if (pack) population_kernel<TersoffPop<NMAX, TWO>, true><<<grid, block, 0, st>>>(ctl, s, 0);
+  else population_kernel<TersoffPop<NMAX, TWO>, false><<<grid, block, 0, st>>>(ctl, s, 0);
 }
 
-// cls: 0 for at most 4 neighbours per atom, 1 for 8, 2 for 16
+// cls: 0 for at most 4 neighbours per atom, 1 for 8, 2 for 16; pack: the
+// packed-item instantiation
 void launch_tersoff_population(const PopCtl& ctl, const DevSet& s, int cls, int grid, int block,
-                               hipStream_t st)
+                               bool pack, hipStream_t st)
 {
   const bool two = ctl.V == 13;
   switch (cls * 2 + (two ? 1 : 0)) {
-  case 0: launch_t<4, false>(ctl, s, grid, block, st); break;
-  case 1: launch_t<4, true>(ctl, s, grid, block, st); break;
-  case 2: launch_t<8, false>(ctl, s, grid, block, st); break;
-  case 3: launch_t<8, true>(ctl, s, grid, block, st); break;
-  case 4: launch_t<16, false>(ctl, s, grid, block, st); break;
-  default: launch_t<16, true>(ctl, s, grid, block, st); break;
+  case 0: launch_t<4, false>(ctl, s, grid, block, pack, st); break;
+  case 1: launch_t<4, true>(ctl, s, grid, block, pack, st); break;
+  case 2: launch_t<8, false>(ctl, s, grid, block, pack, st); break;
+  case 3: launch_t<8, true>(ctl, s, grid, block, pack, st); break;
+  case 4: launch_t<16, false>(ctl, s, grid, block, pack, st); break;
+  default: launch_t<16, true>(ctl, s, grid, block, pack, st); break;
   }
   GPUGA_HIP(hipGetLastError());
 }
 
 template <int NMAX, bool TWO>
-static int occ_t(int block)
+static int occ_t(bool pack, int block)
 {
   int n = 0;
-  GPUGA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-    &n, reinterpret_cast<const void*>(&population_kernel<TersoffPop<NMAX, TWO>>), block, 0));
+  const void* f =
+    pack ? reinterpret_cast<const void*>(&population_kernel<TersoffPop<NMAX, TWO>, true>)
+         : reinterpret_cast<const void*>(&population_kernel<TersoffPop<NMAX, TWO>, false>);
+  GPUGA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, 0));
   return n;
 }
 
-int tersoff_population_occupancy(int cls, bool two, int block)
+int tersoff_population_occupancy(int cls, bool two, bool pack, int block)
 {
   switch (cls * 2 + (two ? 1 : 0)) {
-  case 0: return occ_t<4, false>(block);
-  case 1: return occ_t<4, true>(block);
-  case 2: return occ_t<8, false>(block);
-  case 3: return occ_t<8, true>(block);
-  case 4: return occ_t<16, false>(block);
-  default: return occ_t<16, true>(block);
+  case 0: return occ_t<4, false>(pack, block);
+  case 1: return occ_t<4, true>(pack, block);
+  case 2: return occ_t<8, false>(pack, block);
+  case 3: return occ_t<8, true>(pack, block);
+  case 4: return occ_t<16, false>(pack, block);
+  default: return occ_t<16, true>(pack, block);
   }
 }
 

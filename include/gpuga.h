@@ -28,6 +28,10 @@ typedef struct gpuga_options {
   int nn2b_bf16;  /* 1: NN2B hidden layer on BF16 MFMA; 0: FP32 */
   int nn2b_grid;  /* NN2B tabulation points (default 1024) */
   int verbose;
+  int blocks_per_cu; /* resident workgroups per CU of the population launch;
+                        0 = as many as the occupancy calculator allows */
+  int pack_lanes;    /* 1: a last lane block of <= 32 individuals runs several
+                        atoms per wave (three-body potentials); 0: one atom */
 } gpuga_options;
 
 /* Thread-local text of the last failure (src/error.cu:21-25 prints it). */

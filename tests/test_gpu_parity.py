@@ -115,6 +115,25 @@ def test_population_sizes(G, golden, oracle_case, P):
     np.testing.assert_allclose(got[sel], ref, rtol=1e-5)
 
 
+@pytest.mark.parametrize("name,P", [("si_diamond", 72), ("si_diamond", 65), ("si_diamond", 8),
+                                    ("gaas", 90), ("si_type2", 130)])
+def test_packed_lane_block_matches_unpacked(G, golden, name, P):
+    """A last lane block of <= 32 individuals runs several atoms per wave (its
+    own queue and kernel); its fitness and terms must equal the one-atom-per-
+    wave form (same FP32 formulas, FP64 sums regrouped), for the individuals
+    of that block and the full blocks alike."""
+    print_case("packed lanes", name, P)
+    packed = product(G, golden, name)
+    plain = product(G, golden, name, pack_lanes=0)
+    genes = genes_for(packed.V, P, 300 + P)
+    a, ta = packed.compute(genes, terms=True)
+    b, tb = plain.compute(genes, terms=True)
+    np.testing.assert_allclose(ta, tb, rtol=1e-5)
+    np.testing.assert_allclose(a, b, rtol=1e-5)
+    again, tag = packed.compute(genes, terms=True)
+    assert np.array_equal(ta, tag)
+
+
 def test_replay_is_bit_identical(G, golden):
     print_case("graph replay")
     fit = product(G, golden, "si_diamond")
