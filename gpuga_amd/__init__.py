@@ -38,7 +38,7 @@ class GpugaError(RuntimeError):
 class Options(C.Structure):
     _fields_ = [("backend", C.c_int), ("chunk_cost", C.c_int), ("use_graph", C.c_int),
                 ("nn2b_bf16", C.c_int), ("nn2b_grid", C.c_int), ("verbose", C.c_int),
-                ("blocks_per_cu", C.c_int), ("pack_lanes", C.c_int)]
+                ("blocks_per_cu", C.c_int), ("pack_lanes", C.c_int), ("queues", C.c_int)]
 
 
 def build(jobs: int = 8) -> None:
@@ -227,13 +227,13 @@ class Fitness:
         return ms.value, n.value
 
     def trace(self, genes):
-        """Work-item timeline of one population generation: array [n_items, 9]
+        """Work-item timeline of one population generation: array [n_items, 10]
         of (config, lane block, chunk, atom begin, atom end, phase, wave slot,
-        start ns, end ns)."""
+        start ns, end ns, wave entry ns on the wave's first item else NaN)."""
         g = _f32(genes).reshape(-1, self.V)
         n = C.c_int(0)
         _check(lib().gpuga_fitness_trace(self._h, g.shape[0], _ptr(g), None, C.byref(n)))
-        rows = np.zeros((n.value, 9))
+        rows = np.zeros((n.value, 10))
         _check(lib().gpuga_fitness_trace(self._h, g.shape[0], _ptr(g), _ptr(rows, C.c_double),
                                          C.byref(n)))
         return rows

@@ -52,6 +52,7 @@ FitnessOptions to_options(const gpuga_options* o)
   f.verbose = o->verbose;
   f.blocks_per_cu = o->blocks_per_cu;
   f.pack_lanes = o->pack_lanes;
+  f.queues = o->queues;
   return f;
 }
 
@@ -78,6 +79,7 @@ void gpuga_default_options(gpuga_options* opt)
   opt->verbose = f.verbose;
   opt->blocks_per_cu = f.blocks_per_cu;
   opt->pack_lanes = f.pack_lanes;
+  opt->queues = f.queues;
 }
 
 int gpuga_fitness_create(const char* input_dir, const gpuga_options* opt, gpuga_fitness** out)
@@ -301,8 +303,8 @@ int gpuga_fitness_trace(gpuga_fitness* h, int P, const float* genes, double* row
     need(genes, "genes");
     need(n_items, "n_items");
     const std::vector<double> r = h->f->trace(P, genes);
-    const int n = (int)(r.size() / 9);
-    if (rows) std::memcpy(rows, r.data(), sizeof(double) * 9 * std::min(n, *n_items));
+    const int n = (int)(r.size() / 10);
+    if (rows) std::memcpy(rows, r.data(), sizeof(double) * 10 * std::min(n, *n_items));
     *n_items = n;
   });
 }

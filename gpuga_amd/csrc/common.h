@@ -3,10 +3,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <exception>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -18,6 +21,34 @@ namespace gpuga {
 struct Error : std::runtime_error {
   explicit Error(const std::string& s) : std::runtime_error(s) {}
 };
+
+// f(i) for i in [0, n) on up to 16 host threads, in contiguous blocks; f
+// must write only outputs owned by i, so results do not depend on the
+// thread count.  An exception from any f is rethrown after all have joined.
+template <class F>
+void parallel_for(int n, F&& f)
+{
+  const int hw = (int)std::thread::hardware_concurrency();
+  const int T = std::max(1, std::min({hw > 0 ? hw : 1, 16, n / 16}));
+  if (T <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  std::vector<std::exception_ptr> err(T);
+  for (int t = 0; t < T; ++t)
+    th.emplace_back([&, t] {
+      try {
+        for (int i = (int)((long long)n * t / T); i < (int)((long long)n * (t + 1) / T); ++i)
+          f(i);
+      } catch (...) {
+        err[t] = std::current_exception();
+      }
+    });
+  for (auto& x : th) x.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+}
 
 inline std::string strfmt(const char* fmt, ...)
 {

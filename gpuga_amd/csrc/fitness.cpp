@@ -149,9 +149,6 @@ Fitness::~Fitness()
   plans_.clear();
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
-  if (ev_fork_) (void)hipEventDestroy(ev_fork_);
-  if (ev_join_) (void)hipEventDestroy(ev_join_);
-  if (side_) (void)hipStreamDestroy(side_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -165,11 +162,8 @@ void Fitness::setup()
   GPUGA_HIP(hipGetDeviceProperties(&prop, device_));
   num_cu_ = prop.multiProcessorCount;
   GPUGA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  GPUGA_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
   GPUGA_HIP(hipEventCreate(&ev0_));
   GPUGA_HIP(hipEventCreate(&ev1_));
-  GPUGA_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
-  GPUGA_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
 
   const bool tersoff = spec_.type == POT_TERSOFF1 || spec_.type == POT_TERSOFF2;
   if (spec_.type == POT_RI)
@@ -402,7 +396,7 @@ size_t Fitness::device_bytes() const
   for (auto& kv : plans_) {
     const PopPlan& p = *kv.second;
     b += p.items.bytes() + p.part.bytes() + p.err.bytes() + p.bondf.bytes() + p.genes.bytes() +
-         p.fitness.bytes() + p.terms.bytes() + p.counters.bytes() + p.nn2b_table.bytes();
+         p.out.bytes() + p.gpart.bytes() + p.counters.bytes() + p.nn2b_table.bytes();
   }
   return b;
 }
@@ -518,35 +512,106 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
       eval.push_back(Costed{WorkItem{ch.c, k, ch.chunk, ch.a0, ch.a1, ch.cls, 0, G},
                             ch.cost * steps / n});
     }
-  std::stable_sort(eval.begin(), eval.end(),
-                   [](const Costed& a, const Costed& b) { return a.cost > b.cost; });
-  // packed items form a queue of their own, served by a second kernel
-  // instantiation (its per-lane geometry loads need more registers)
-  std::vector<WorkItem> items, packed;
-  items.reserve(chunks.size() * K * 2);
-  for (const Costed& e : eval) (e.w.groups > 1 ? packed : items).push_back(e.w);
-  // three-body force configurations: force-assembly items after every
-  // evaluation item of their queue (the order the kernel's dependency waits
-  // rely on)
+  // three-body force configurations: one force-assembly item per chunk and
+  // lane block
+  std::vector<Costed> assemble;
   if (tersoff)
     for (const Chunk& ch : chunks)
       if (ch.c < ts_.Nc_force)
         for (int k = 0; k < K; ++k) {
           const int G = k == K - 1 ? G_last : 1;
-          (G > 1 ? packed : items).push_back(WorkItem{ch.c, k, ch.chunk, ch.a0, ch.a1, ch.cls, 1, G});
+          const int n = ch.a1 - ch.a0, steps = (n + G - 1) / G;
+          assemble.push_back(
+            Costed{WorkItem{ch.c, k, ch.chunk, ch.a0, ch.a1, ch.cls, 1, G}, 8LL * steps});
         }
-  pl.n_items = (int)items.size();
-  pl.n_packed = (int)packed.size();
+  pl.n_items = (int)(eval.size() + assemble.size());
+  const int waves_per_block = pl.block / WAVE;
+  {
+    int occ = opt_.blocks_per_cu;
+    if (occ <= 0) {
+      switch (spec_.type) {
+      case POT_RI: occ = ri_population_occupancy(pl.block); break;
+      case POT_NN2B: occ = nn2b_population_occupancy(pl.block); break;
+      default:
+        occ = tersoff_population_occupancy(nn_class(nl_.max_nn), spec_.type == POT_TERSOFF2,
+                                           G_last > 1, pl.block);
+        break;
+      }
+      occ = std::max(occ, 1);
+    }
+    const int need = (pl.n_items + waves_per_block - 1) / waves_per_block;
+    pl.grid = std::max(1, std::min(need, num_cu_ * occ));
+  }
+
+  // Queues, kQueuesPerXcd per XCD (queue q is served first by XCD
+  // q / kQueuesPerXcd): whole configurations, balanced by cost (largest
+  // configuration to the least loaded queue).  Within a queue, phase-0 items
+  // largest chunk first, the lane blocks of a chunk together; a force-assembly
+  // item follows the last phase-0 item of its configuration by about one
+  // item per resident wave of the queue.
+  const int nq = std::max(1, std::min(opt_.queues > 0 ? opt_.queues : kQueues,
+                                      pl.grid * waves_per_block));
+  std::vector<long long> cfg_cost(Nc, 0);
+  for (const Costed& e : eval) cfg_cost[e.w.c] += e.cost;
+  for (const Costed& e : assemble) cfg_cost[e.w.c] += e.cost;
+  std::vector<int> by_cost(Nc), cfg_q(Nc, 0);
+  std::iota(by_cost.begin(), by_cost.end(), 0);
+  std::stable_sort(by_cost.begin(), by_cost.end(),
+                   [&](int a, int b) { return cfg_cost[a] > cfg_cost[b]; });
+  std::vector<long long> load(nq, 0);
+  for (int c : by_cost) {
+    const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    cfg_q[c] = q;
+    load[q] += cfg_cost[c];
+  }
+  // eval is in chunk order (largest chunk first, its lane blocks together):
+  // a packed item stays beside its siblings, so the packed block's
+  // force-assembly items do not all wait at the end of the launch
+  const long long slack = std::max(1, pl.grid * waves_per_block / nq);
+  std::vector<WorkItem> items;
+  items.reserve(pl.n_items);
+  std::vector<int> qstart(nq + 1, 0);
+  for (int q = 0; q < nq; ++q) {
+    qstart[q] = (int)items.size();
+    // (position key, tie, item): phase-0 items at their rank in the queue,
+    // assembly items at rank of their configuration's last phase-0 item +
+    // slack (after it in any case)
+    struct Keyed {
+      long long key;
+      int tie;
+      WorkItem w;
+    };
+    std::vector<Keyed> qi;
+    std::vector<long long> last(Nc, -1);
+    long long pos = 0;
+    for (const Costed& e : eval)
+      if (cfg_q[e.w.c] == q) {
+        qi.push_back(Keyed{pos, 0, e.w});
+        last[e.w.c] = pos++;
+      }
+    int tie = 1;
+    for (const Costed& e : assemble)
+      if (cfg_q[e.w.c] == q) qi.push_back(Keyed{last[e.w.c] + slack, tie++, e.w});
+    std::stable_sort(qi.begin(), qi.end(), [](const Keyed& a, const Keyed& b) {
+      return a.key != b.key ? a.key < b.key : a.tie < b.tie;
+    });
+    for (const Keyed& x : qi) items.push_back(x.w);
+  }
+  qstart[nq] = (int)items.size();
+  pl.nq = nq;
 
   hipStream_t st = stream_;
   pl.items.upload(items, st);
-  pl.items_packed.upload(packed, st);
+  pl.qstart.upload(qstart, st);
   pl.cfg_nchunks.upload(nch, st);
   pl.cfg_slot0.upload(slot0, st);
   pl.part.resize((size_t)std::max(slots, 1) * K * 8 * WAVE);
   pl.err.resize((size_t)K * 3 * Nc * WAVE);
-  pl.terms.resize(3 * (size_t)P);
-  pl.fitness.resize(P);
+  // results in one buffer, so one copy returns them: terms [P][3] (FP64),
+  // fitness [P] (FP32), status word
+  const size_t out_doubles = 3 * (size_t)P + ((size_t)P + 1 + 1) / 2;
+  pl.out.resize(out_doubles);
+  pl.out.zero(st);
   pl.genes.resize((size_t)P * spec_.V);
   const int B_force =
     ts_.N_force > 0 ? nl_.start[ts_.N_force - 1] + nl_.NN[ts_.N_force - 1] : 0;
@@ -555,18 +620,19 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   const int groups = (Nc + CFG_GROUP - 1) / CFG_GROUP;
   pl.gpart.resize((size_t)K * groups * 3 * WAVE);
   // counters: cnt [Nc*K] | cnt2 [Nc*K] | grp_done [K*groups] | blk_done [K] |
-  // fetch | status | fetch of the packed queue
-  pl.counters.resize((size_t)2 * Nc * K + (size_t)K * groups + K + 3);
+  // (pad to 4 KiB) fetch [nq, kFetchStride apart] | done
+  const size_t fetch0 = ((size_t)2 * Nc * K + (size_t)K * groups + K + 1023) / 1024 * 1024;
+  pl.counters.resize(fetch0 + (size_t)nq * kFetchStride + 1);
   pl.counters.zero(st);
   pl.h_genes.resize((size_t)P * spec_.V);
-  pl.h_fitness.resize(P);
-  pl.h_terms.resize(3 * (size_t)P);
-  pl.h_status.resize(1);
-  pl.h_status[0] = 0u;
+  pl.h_out.resize(out_doubles);
+  std::memset(pl.h_out.get(), 0, sizeof(double) * out_doubles);
 
   PopCtl& c = pl.ctl;
   c.items = pl.items.get();
   c.n_items = pl.n_items;
+  c.nq = nq;
+  c.qstart = pl.qstart.get();
   c.K = K;
   c.P = P;
   c.V = spec_.V;
@@ -584,39 +650,13 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   c.cnt2 = ctr + (size_t)Nc * K;
   c.grp_done = ctr + (size_t)2 * Nc * K;
   c.blk_done = c.grp_done + (size_t)K * groups;
-  c.fetch = c.blk_done + K;
-  c.status = c.fetch + 1;
+  c.fetch = ctr + fetch0;
+  c.done = c.fetch + (size_t)nq * kFetchStride;
+  c.terms = pl.out.get();
+  c.fitness = reinterpret_cast<float*>(c.terms + 3 * (size_t)P);
+  c.status = reinterpret_cast<unsigned*>(c.fitness + P);
   c.gpart = pl.gpart.get();
-  c.fitness = pl.fitness.get();
-  c.terms = pl.terms.get();
   c.norm = norm();
-
-  PopCtl& cp = pl.ctl_packed;
-  cp = c;
-  cp.items = pl.items_packed.get();
-  cp.n_items = pl.n_packed;
-  cp.fetch = c.status + 1;
-
-  const int cls = nn_class(nl_.max_nn);
-  const bool two = spec_.type == POT_TERSOFF2;
-  int occ = opt_.blocks_per_cu, occ_packed = opt_.blocks_per_cu;
-  if (occ <= 0) {
-    switch (spec_.type) {
-    case POT_RI: occ = ri_population_occupancy(pl.block); break;
-    case POT_NN2B: occ = nn2b_population_occupancy(pl.block); break;
-    default:
-      occ = tersoff_population_occupancy(cls, two, false, pl.block);
-      occ_packed = tersoff_population_occupancy(cls, two, true, pl.block);
-      break;
-    }
-    occ = std::max(occ, 1);
-    occ_packed = std::max(occ_packed, 1);
-  }
-  const int waves_per_block = pl.block / WAVE;
-  const int need = (pl.n_items + waves_per_block - 1) / waves_per_block;
-  pl.grid = std::max(1, std::min(need, num_cu_ * occ));
-  const int need_packed = (pl.n_packed + waves_per_block - 1) / waves_per_block;
-  pl.grid_packed = std::max(1, std::min(need_packed, num_cu_ * occ_packed));
   GPUGA_HIP(hipStreamSynchronize(st));
 }
 
@@ -651,22 +691,9 @@ void Fitness::launch_population(PopPlan& pl, hipStream_t st)
     break;
   }
   default: {
-    // the packed queue runs on a second stream, forked from and joined back
-    // into the launch stream (two parallel nodes when captured), so its
-    // waves fill the slots the main launch frees as it drains
-    const int cls = nn_class(nl_.max_nn);
-    if (pl.n_packed > 0) {
-      GPUGA_HIP(hipEventRecord(ev_fork_, st));
-      GPUGA_HIP(hipStreamWaitEvent(side_, ev_fork_, 0));
-    }
-    if (pl.n_items > 0)
-      launch_tersoff_population(pl.ctl, ds_.view, cls, pl.grid, pl.block, false, st);
-    if (pl.n_packed > 0) {
-      launch_tersoff_population(pl.ctl_packed, ds_.view, cls, pl.grid_packed, pl.block, true,
-                                side_);
-      GPUGA_HIP(hipEventRecord(ev_join_, side_));
-      GPUGA_HIP(hipStreamWaitEvent(st, ev_join_, 0));
-    }
+    // the instantiation with the packed path only when the plan has packed items
+    launch_tersoff_population(pl.ctl, ds_.view, nn_class(nl_.max_nn), pl.grid, pl.block,
+                              pl.groups_last > 1, st);
     break;
   }
   }
@@ -693,11 +720,8 @@ void Fitness::compute_async(int P, const float* genes)
     launch_population(pl, st);
   }
   if (timing_) GPUGA_HIP(hipEventRecord(ev1_, st));
-  GPUGA_HIP(hipMemcpyAsync(pl.h_fitness.get(), pl.fitness.get(), sizeof(float) * P,
-                           hipMemcpyDeviceToHost, st));
-  GPUGA_HIP(hipMemcpyAsync(pl.h_terms.get(), pl.terms.get(), sizeof(double) * 3 * P,
-                           hipMemcpyDeviceToHost, st));
-  GPUGA_HIP(hipMemcpyAsync(pl.h_status.get(), pl.ctl.status, sizeof(unsigned),
+  GPUGA_HIP(hipMemcpyAsync(pl.h_out.get(), pl.out.get(),
+                           sizeof(double) * 3 * P + sizeof(float) * P + sizeof(unsigned),
                            hipMemcpyDeviceToHost, st));
   pending_P_ = P;
 }
@@ -715,15 +739,18 @@ void Fitness::compute_wait(float* fitness, double* terms)
     ++kernel_launches_;
   }
   pending_P_ = 0;
-  if (pl.h_status[0] != 0u) {
+  const double* h_terms = pl.h_out.get();
+  const float* h_fitness = reinterpret_cast<const float*>(h_terms + 3 * (size_t)P);
+  const unsigned h_status = *reinterpret_cast<const unsigned*>(h_fitness + P);
+  if (h_status != 0u) {
     // a dependency wait of the population kernel ran out: the results of
     // this launch are not valid; clear the flag and report
     GPUGA_HIP(hipMemsetAsync(pl.ctl.status, 0, sizeof(unsigned), stream_));
     GPUGA_HIP(hipStreamSynchronize(stream_));
     fail("population kernel: force-assembly wait timed out; results discarded");
   }
-  if (fitness) std::memcpy(fitness, pl.h_fitness.get(), sizeof(float) * P);
-  if (terms) std::memcpy(terms, pl.h_terms.get(), sizeof(double) * 3 * P);
+  if (fitness) std::memcpy(fitness, h_fitness, sizeof(float) * P);
+  if (terms) std::memcpy(terms, h_terms, sizeof(double) * 3 * P);
 }
 
 void Fitness::compute(int P, const float* genes, float* fitness, double* terms)
@@ -856,36 +883,32 @@ std::vector<double> Fitness::trace(int P, const float* genes)
 {
   PopPlan& pl = plan(P);
   hipStream_t st = stream_;
-  // main queue, then the packed queue (whose wave slots follow the main ones)
-  const size_t n0 = pl.n_items, n = n0 + pl.n_packed;
+  const size_t n = pl.n_items;
   DeviceBuffer<unsigned long long> buf(std::max<size_t>(4 * n, 1));
   buf.zero(st);
   std::vector<WorkItem> items(n);
-  pl.items.download(items.data(), n0, st);
-  pl.items_packed.download(items.data() + n0, pl.n_packed, st);
+  pl.items.download(items.data(), n, st);
   GPUGA_HIP(hipMemcpyAsync(pl.genes.get(), genes, sizeof(float) * (size_t)P * spec_.V,
                            hipMemcpyHostToDevice, st));
   pl.ctl.trace = buf.get();
-  pl.ctl_packed.trace = buf.get() + 4 * n0;
   launch_population(pl, st);
   pl.ctl.trace = nullptr;
-  pl.ctl_packed.trace = nullptr;
   std::vector<unsigned long long> t(4 * n);
   buf.download(t.data(), t.size(), st);
   GPUGA_HIP(hipStreamSynchronize(st));
-  for (size_t i = n0; i < n; ++i) t[4 * i] += (unsigned long long)pl.grid * (pl.block / WAVE);
   int khz = 0;
   GPUGA_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_));
   const double ns_per_tick = 1e6 / std::max(khz, 1);
   unsigned long long t0 = ~0ull;
   for (size_t i = 0; i < n; ++i) t0 = std::min(t0, t[4 * i + 1]);
+  auto rel = [&](unsigned long long x) { return ((double)x - (double)t0) * ns_per_tick; };
   std::vector<double> rows;
-  rows.reserve(9 * n);
+  rows.reserve(10 * n);
   for (size_t i = 0; i < n; ++i) {
     const WorkItem& w = items[i];
     rows.insert(rows.end(), {(double)w.c, (double)w.k, (double)w.chunk, (double)w.a0,
-                             (double)w.a1, (double)w.phase, (double)t[4 * i],
-                             (t[4 * i + 1] - t0) * ns_per_tick, (t[4 * i + 2] - t0) * ns_per_tick});
+                             (double)w.a1, (double)w.phase, (double)t[4 * i], rel(t[4 * i + 1]),
+                             rel(t[4 * i + 2]), t[4 * i + 3] ? rel(t[4 * i + 3]) : NAN});
   }
   return rows;
 }

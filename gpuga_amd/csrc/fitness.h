@@ -22,6 +22,7 @@ struct FitnessOptions {
   int verbose = 0;
   int blocks_per_cu = 0;  // 0 = from the occupancy calculator
   int pack_lanes = 1;     // pack a last lane block of <= 32 individuals
+  int queues = 0;         // work queues of the population launch; 0 = one per XCD
 };
 
 // Device-resident copy of one training set and its geometry tables.
@@ -52,16 +53,17 @@ std::unique_ptr<Potential> make_potential(const PotentialSpec& spec, const Devic
 struct PopPlan {
   int P = 0, K = 0, n_items = 0, grid = 0, block = 256, chunk_cost = 0;
   int groups_last = 1; // lane groups of the last lane block's items
-  int n_packed = 0, grid_packed = 0; // packed-item queue (its own kernel)
-  DeviceBuffer<WorkItem> items, items_packed;
+  int nq = 1;          // work queues
+  DeviceBuffer<WorkItem> items;
+  DeviceBuffer<int> qstart;
   DeviceBuffer<int> cfg_nchunks, cfg_slot0;
-  DeviceBuffer<double> part, err, gpart, terms;
-  DeviceBuffer<float> bondf, genes, fitness, nn2b_table;
-  DeviceBuffer<unsigned> counters; // [Nc*K] cnt | [K] blk_done | [1] fetch
-  PinnedBuffer<float> h_genes, h_fitness;
-  PinnedBuffer<double> h_terms;
-  PinnedBuffer<unsigned> h_status;
-  PopCtl ctl{}, ctl_packed{};
+  DeviceBuffer<double> part, err, gpart;
+  DeviceBuffer<double> out; // terms [P][3] | fitness [P] (float) | status (unsigned)
+  DeviceBuffer<float> bondf, genes, nn2b_table;
+  DeviceBuffer<unsigned> counters; // cnt | cnt2 | grp_done | blk_done | fetch
+  PinnedBuffer<float> h_genes;
+  PinnedBuffer<double> h_out;
+  PopCtl ctl{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~PopPlan();
@@ -86,9 +88,10 @@ public:
   void compute_wait(float* fitness, double* terms);
   // Tune the work-item size for population size P.
   int autotune(int P);
-  // One generation with the item timeline recorded: rows of 9 values per
+  // One generation with the item timeline recorded: rows of 10 values per
   // work item (config, lane block, chunk, atom begin, atom end, phase, wave
-  // slot, start ns, end ns; times relative to the first start).
+  // slot, start ns, end ns, entry ns of the wave for its first item else
+  // NaN; times relative to the first start).
   std::vector<double> trace(int P, const float* genes);
   // Work-item size from the training set alone (no timing; deterministic).
   int heuristic_chunk_cost(int P) const;
@@ -138,8 +141,6 @@ private:
   DeviceBuffer<float> d_f_, d_virial_, d_pe_, d_params_, d_fit1_, d_pmin_, d_pmax_;
   DeviceBuffer<double> d_cerr_, d_terms1_;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
-  hipStream_t side_ = nullptr; // second stream of the packed-item launch
-  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
   bool timing_ = false;
   double kernel_ms_ = 0.0;
   long kernel_launches_ = 0;

@@ -9,16 +9,21 @@
 // differ per lane.  Per-configuration energy and virial are accumulated per
 // lane in FP64, so no cross-lane reduction is ever needed.
 //
-// Schedule.  Work items are pulled from a global counter by the resident
-// waves.  Phase-0 items (configuration chunk x lane block, sorted by cost,
-// largest first) evaluate the potential.  For three-body potentials the force
-// on an atom needs the partial forces of its neighbours, so phase-0 items of
-// force configurations only publish per-bond forces, and phase-1 items (queued
-// after every phase-0 item) assemble F_i - F_ref,i for their chunk once all
-// chunks of their configuration have published.  Because every phase-0 item
-// is handed out before any phase-1 item and phase-0 work never waits, each
-// such wait ends; it is bounded anyway, and a timeout is reported through a
-// status word instead of hanging.
+// Schedule.  The resident waves pull work items from queues (a few per XCD;
+// each configuration belongs to one queue, so its atom records and per-bond
+// forces mostly stay in one L2), each queue with its own counter: a wave
+// serves a queue of its own XCD and, once that is empty, the others.
+// Phase-0 items (configuration chunk x lane block, largest chunk first, the
+// lane blocks of a chunk together) evaluate the potential.  For three-body potentials the force on an atom needs the
+// partial forces of its neighbours, so phase-0 items of force configurations
+// only publish per-bond forces, and phase-1 items assemble F_i - F_ref,i for
+// their chunk once all chunks of their configuration have published.  A
+// phase-1 item sits in its configuration's queue after every phase-0 item of
+// that configuration (about one item per resident wave of the queue later,
+// so that they have finished by then).  Items of a queue are handed out in
+// queue order to running waves and phase-0 work never waits, so every wait
+// ends; it is bounded anyway, and a timeout is reported through a status
+// word instead of hanging.
 //
 // Packed lane block.  When the last lane block holds at most 32 individuals
 // (P mod 64), its items split the wave into G = 64 / W groups of W lanes
@@ -47,6 +52,12 @@
 namespace gpuga {
 
 constexpr int CFG_GROUP = 64; // configurations per reduction group
+constexpr int kQueuesPerXcd = 4; // work queues per XCD
+constexpr int kXcds = 8;         // XCDs of an MI355X
+constexpr int kQueues = kQueuesPerXcd * kXcds;
+// distance of two queue counters in 32-bit words (4 KiB + 256 B: distinct
+// lines and, with the address interleave, distinct memory channels)
+constexpr int kFetchStride = 1088;
 
 struct WorkItem {
   int c;      // configuration
@@ -59,8 +70,10 @@ struct WorkItem {
 };
 
 struct PopCtl {
-  const WorkItem* items;
+  const WorkItem* items;  // the queues, one after another
   int n_items;
+  int nq;                 // queues
+  const int* qstart;      // [nq + 1] first item of each queue
   int K;                  // lane blocks
   int P, V;
   const int* cfg_nchunks; // [Nc]
@@ -77,7 +90,8 @@ struct PopCtl {
   unsigned* cnt2;         // [Nc][K]  phase-1 chunks done
   unsigned* grp_done;     // [K][groups]
   unsigned* blk_done;     // [K]
-  unsigned* fetch;        // [1]
+  unsigned* fetch;        // [nq] items handed out per queue, kFetchStride apart
+  unsigned* done;         // [1] waves finished
   unsigned* status;       // [1] non-zero: a dependency wait timed out
   float* fitness;         // [P]
   double* terms;          // [P][3]
@@ -231,6 +245,40 @@ __device__ inline bool fold_chunks(const PopCtl& ctl, unsigned* counter, int c, 
   return true;
 }
 
+// The XCD this wave runs on (hardware register; affects only which queue a
+// wave serves first, never a result).
+__device__ __forceinline__ unsigned xcc_id()
+{
+  unsigned x;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+  return x & 0xfu;
+}
+
+// Next item for this wave: from queue q while it has items, then from the
+// following queues; -1 when every queue is empty.  The current queue is
+// fetched from directly (one atomic); another queue's counter is read first,
+// so an empty queue costs a load, not an atomic.  Counters are kFetchStride
+// words apart, each in its own memory channel's lines.
+__device__ inline int next_item(const PopCtl& ctl, int& q)
+{
+  for (int tries = 0; tries < ctl.nq; ++tries) {
+    unsigned* f = ctl.fetch + (long)q * kFetchStride;
+    const unsigned q0 = (unsigned)ldu(ctl.qstart, q), len = (unsigned)ldu(ctl.qstart, q + 1) - q0;
+    bool open = tries == 0;
+    if (!open) {
+      unsigned seen = 0;
+      if (__lane_id() == 0) seen = cload(f);
+      open = (unsigned)wave_first((int)seen) < len;
+    }
+    if (open) {
+      const unsigned it = wave_atomic_add(f, 1u);
+      if (it < len) return (int)(q0 + it);
+    }
+    q = q + 1 == ctl.nq ? 0 : q + 1;
+  }
+  return -1;
+}
+
 // Sum over the G lane groups of a packed item: lanes l, l ^ W, l ^ 2W, ...
 // (fixed order; every lane ends with its column's total).
 __device__ inline double group_sum(double x, int W)
@@ -250,32 +298,45 @@ __device__ inline double group_sum(double x, int W)
 //   static void chunk_packed(set, q, item, ctl, ex, col, g, G, Acc&)
 //   static double force_pass_packed(set, ctl, a0, a1, k, col, g, G)
 //                                                   the same for lane group g
-// PACK selects the instantiation that serves the packed-item queue; the
-// ordinary one never contains the packed code, so its register budget is
-// unaffected.
+// PACK selects the instantiation that also serves packed items (chosen per
+// item); the other one never contains the packed code.  Pot::kMinWaves is
+// the occupancy the register allocator must keep (waves per SIMD), so the
+// packed code cannot cost the common path a wave.
 template <class Pot, bool PACK = false>
-__global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
-                                                         typename Pot::Extra ex)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Pot::kMinWaves)))
+population_kernel(PopCtl ctl, DevSet set, typename Pot::Extra ex)
 {
   const int lane = __lane_id();
   const unsigned total_waves = gridDim.x * (blockDim.x / WAVE);
   const unsigned wave_slot = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
   const int Nc = set.Nc;
   unsigned prev = 0xffffffffu;
+  const unsigned long long t_entry = ctl.trace ? wall_clock64() : 0ull;
+  // home queue: one of the queues of this wave's XCD (workgroups of one XCD
+  // spread over its queues)
+  int q = (int)((xcc_id() * kQueuesPerXcd + (blockIdx.x / kXcds) % kQueuesPerXcd) %
+                (unsigned)ctl.nq);
   for (;;) {
-    const unsigned it = wave_atomic_add(ctl.fetch, 1u);
+    const int it = next_item(ctl, q);
     if (ctl.trace && lane == 0) {
       const unsigned long long now = wall_clock64();
       if (prev != 0xffffffffu) ctl.trace[4 * (long)prev + 2] = now;
-      if (it < (unsigned)ctl.n_items) {
+      if (it >= 0) {
         ctl.trace[4 * (long)it + 0] = wave_slot;
         ctl.trace[4 * (long)it + 1] = now;
+        // the wave's entry time on its first item (fetch latency), else 0
+        ctl.trace[4 * (long)it + 3] = prev == 0xffffffffu ? t_entry : 0ull;
       }
-      prev = it;
+      prev = (unsigned)it;
     }
-    if (it >= (unsigned)ctl.n_items) {
-      // The last fetch of the launch returns the counter to zero.
-      if (it == (unsigned)ctl.n_items + total_waves - 1u && lane == 0) *ctl.fetch = 0u;
+    if (it < 0) {
+      // Every queue is empty.  The last wave to finish returns the queue
+      // counters to zero (all other waves have stopped fetching by then).
+      const unsigned d = wave_atomic_add(ctl.done, 1u);
+      if (d == total_waves - 1u && lane == 0) {
+        for (int j = 0; j < ctl.nq; ++j) cstore(ctl.fetch + (long)j * kFetchStride, 0u);
+        cstore(ctl.done, 0u);
+      }
       break;
     }
     const WorkItem item = ldu(ctl.items, it);
@@ -287,16 +348,19 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc.v[u] = 0.0;
 
-    constexpr bool packed = PACK && Pot::kPacked;
+    constexpr bool can_pack = PACK && Pot::kPacked;
+    const bool packed = can_pack && item.groups > 1;
     if (item.phase == 0) {
-      if constexpr (packed) {
-        const int G = item.groups, W = WAVE / G;
-        const int col = lane & (W - 1);
-        const typename Pot::Lane q = Pot::make(ctl, min(k * WAVE + col, ctl.P - 1));
-        Pot::chunk_packed(set, q, item, ctl, ex, col, lane / W, G, acc);
-        if (!(Pot::kForcePass && is_force)) {
+      if (packed) {
+        if constexpr (can_pack) {
+          const int G = item.groups, W = WAVE / G;
+          const int col = lane & (W - 1);
+          const typename Pot::Lane q = Pot::make(ctl, min(k * WAVE + col, ctl.P - 1));
+          Pot::chunk_packed(set, q, item, ctl, ex, col, lane / W, G, acc);
+          if (!(Pot::kForcePass && is_force)) {
 #pragma unroll
-          for (int u = 0; u < 8; ++u) acc.v[u] = group_sum(acc.v[u], W);
+            for (int u = 0; u < 8; ++u) acc.v[u] = group_sum(acc.v[u], W);
+          }
         }
       } else {
         // individual of this lane; lanes past P repeat the last one, write nothing
@@ -317,10 +381,13 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
       if (!wave_wait_count(cnt, (unsigned)nch)) {
         if (lane == 0) atomicOr(ctl.status, 1u);
       }
-      if constexpr (packed) {
-        const int G = item.groups, W = WAVE / G;
-        acc.v[7] = group_sum(
-          Pot::force_pass_packed(set, ctl, item.a0, item.a1, k, lane & (W - 1), lane / W, G), W);
+      if (packed) {
+        if constexpr (can_pack) {
+          const int G = item.groups, W = WAVE / G;
+          acc.v[7] = group_sum(
+            Pot::force_pass_packed(set, ctl, item.a0, item.a1, k, lane & (W - 1), lane / W, G),
+            W);
+        }
       } else {
         acc.v[7] = Pot::force_pass(set, ctl, item.a0, item.a1, k, lane);
       }

@@ -243,22 +243,31 @@ struct TersoffPop {
                               Acc& acc)
   {
     const float4* R = s.rec + (long)i * RS;
-    // Every load is unconditional (records are zero-padded to the bound), so
-    // the loads of a row issue together and are waited for once, instead of
-    // one load and wait per guarded use.
+    // Uniform records: every load is unconditional (records are zero-padded
+    // to the bound), so the loads of a row issue together and are waited for
+    // once, instead of one load and wait per guarded use.  Per-lane records
+    // (packed items) are loaded at each use instead, which keeps them out of
+    // VGPRs.
     const int4 hd = ldx<U>(reinterpret_cast<const int4*>(R), 0);
+    constexpr bool PRE = U && NMAX <= 8; // preload rows (SGPR budget allows)
     float d[NMAX], dinv[NMAX];
+    if constexpr (PRE) {
 #pragma unroll
-    for (int u = 0; u < NMAX; ++u) {
-      d[u] = ldx<U>(R, 1 + u).w;
-      dinv[u] = ldx<U>(reinterpret_cast<const float*>(R + INV), u);
+      for (int u = 0; u < NMAX; ++u) {
+        d[u] = ldx<U>(R, 1 + u).w;
+        dinv[u] = ldx<U>(reinterpret_cast<const float*>(R + INV), u);
+      }
     }
+    auto dist = [&](int u) { return PRE ? d[u] : ldx<U>(R, 1 + u).w; };
+    auto dist_inv = [&](int u) {
+      return PRE ? dinv[u] : ldx<U>(reinterpret_cast<const float*>(R + INV), u);
+    };
     const int nn = hd.x, unl = hd.y;
     float fc[NMAX], fcp[NMAX], fcb[NMAX], A[NMAX], B[NMAX], pf[NMAX];
 #pragma unroll
     for (int u = 0; u < NMAX; ++u) {
       if (u < nn) {
-        tersoff_fc(q, d[u], fc[u], fcp[u]);
+        tersoff_fc(q, dist(u), fc[u], fcp[u]);
         fcb[u] = fc[u] * q.beta;
       }
     }
@@ -267,14 +276,16 @@ struct TersoffPop {
     for (int u = 0; u < NMAX; ++u) {
       if (u < nn) {
         float cs[NMAX];
+        if constexpr (PRE) {
 #pragma unroll
-        for (int v = 0; v < NMAX; ++v)
-          if (v != u) cs[v] = ldx<U>(R, 1 + NMAX + u * NMAX + v).x;
+          for (int v = 0; v < NMAX; ++v)
+            if (v != u) cs[v] = ldx<U>(R, 1 + NMAX + u * NMAX + v).x;
+        }
         float zeta = 0.0f;
 #pragma unroll
         for (int v = 0; v < NMAX; ++v) {
           if (v < nn && v != u) {
-            const float x = cs[v] - q.h;
+            const float x = (PRE ? cs[v] : ldx<U>(R, 1 + NMAX + u * NMAX + v).x) - q.h;
             zeta = fmaf(fcb[v], x * x, zeta);
           }
         }
@@ -282,11 +293,12 @@ struct TersoffPop {
         tersoff_bond_order(q, zeta, b, bp);
         const int k = TWO ? ((unl >> u) & 1) : 0;
         float fr, frp, fa, fap;
-        tersoff_pair(q, k, d[u], fr, frp, fa, fap);
+        tersoff_pair(q, k, dist(u), fr, frp, fa, fap);
         const float rep = fr - b * fa;
         e = fmaf(fc[u], rep, e);
-        pf[u] = (fcp[u] * rep + fc[u] * (frp - b * fap)) * dinv[u];
-        fcp[u] *= dinv[u];
+        const float di = dist_inv(u);
+        pf[u] = (fcp[u] * rep + fc[u] * (frp - b * fap)) * di;
+        fcp[u] *= di;
         A[u] = bp * fa;
         B[u] = A[u] * fc[u];
       }
@@ -300,14 +312,16 @@ struct TersoffPop {
         //   dsum = sum_k B_k (h - cos)^2                     (times beta)
         // where (A_j + A_k) fc_k = A_j fc_k + B_k
         float4 row[NMAX];
+        if constexpr (PRE) {
 #pragma unroll
-        for (int v = 0; v < NMAX; ++v)
-          if (v != u) row[v] = ldx<U>(R, 1 + NMAX + u * NMAX + v);
+          for (int v = 0; v < NMAX; ++v)
+            if (v != u) row[v] = ldx<U>(R, 1 + NMAX + u * NMAX + v);
+        }
         float cx = 0.0f, cy = 0.0f, cz = 0.0f, dsum = 0.0f;
 #pragma unroll
         for (int v = 0; v < NMAX; ++v) {
           if (v < nn && v != u) {
-            const float4 t = row[v];
+            const float4 t = PRE ? row[v] : ldx<U>(R, 1 + NMAX + u * NMAX + v);
             const float x = t.x - q.h;
             const float w = fmaf(A[u], fc[v], B[v]) * x;
             cx = fmaf(w, t.y, cx);
@@ -377,6 +391,9 @@ struct TersoffPop {
   // Packed item: lane group g takes the atoms a0 + g, a0 + g + G, ...; the
   // per-bond forces go to lane column col (the lane's individual).
   static constexpr bool kPacked = true;
+  // five waves per SIMD at NMAX = 4 (the common path needs 92 VGPRs; the
+  // packed path must fit beside it); no constraint for the larger bounds
+  static constexpr int kMinWaves = NMAX == 4 ? 5 : 1;
   __device__ static void chunk_packed(const DevSet& s, const TLane& q, const WorkItem& it,
                                       const PopCtl& ctl, int, int col, int g, int G, Acc& acc)
   {
@@ -400,12 +417,14 @@ struct TersoffPop {
     const float* fblk = ctl.bondf + (size_t)k * ctl.B_force * 3 * WAVE + lane;
     const long N = s.N;
     double err = 0.0;
-    // two atoms per step, every load unconditional, so all of them are in
-    // flight together
-    for (int i = a0; i < a1; i += 2) {
-      float d[2][3];
+    // four atoms per step, every load unconditional, so all of them are in
+    // flight together (the loads go to memory: their latency, not their
+    // number, sets the time)
+    constexpr int STEP = 4;
+    for (int i = a0; i < a1; i += STEP) {
+      float d[STEP][3];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < STEP; ++h) {
         const int ii = min(i + h, a1 - 1);
         const int4 hd = ldu(reinterpret_cast<const int4*>(s.rec + (long)ii * RS), 0);
         const float* own = fblk + ((size_t)s.bond_force + ii) * 3 * WAVE;
@@ -423,9 +442,11 @@ struct TersoffPop {
         d[h][1] = fy - ldu(s.force_ref, ii + N);
         d[h][2] = fz - ldu(s.force_ref, ii + 2 * N);
       }
-      err += (double)d[0][0] * d[0][0] + (double)d[0][1] * d[0][1] + (double)d[0][2] * d[0][2];
-      if (i + 1 < a1)
-        err += (double)d[1][0] * d[1][0] + (double)d[1][1] * d[1][1] + (double)d[1][2] * d[1][2];
+#pragma unroll
+      for (int h = 0; h < STEP; ++h)
+        if (i + h < a1)
+          err += (double)d[h][0] * d[h][0] + (double)d[h][1] * d[h][1] +
+                 (double)d[h][2] * d[h][2];
     }
     return err;
   }

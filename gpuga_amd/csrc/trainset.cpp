@@ -291,7 +291,7 @@ NeighborList build_neighbors(const TrainSet& ts, float cutoff)
   nl.NN.assign(N, 0);
   nl.start.assign(N, 0);
   std::vector<std::vector<int>> tmp(N);
-  for (int c = 0; c < ts.Nc; ++c) {
+  parallel_for(ts.Nc, [&](int c) {
     const float* h = &ts.h[18 * (size_t)c];
     const int b = ts.Na_sum[c], e = b + ts.Na[c];
     for (int i = b; i < e; ++i) {
@@ -302,7 +302,7 @@ NeighborList build_neighbors(const TrainSet& ts, float cutoff)
         if (mic_d2(h, x, y, z) < rc2) tmp[i].push_back(j);
       }
     }
-  }
+  });
   long long total = 0;
   for (int i = 0; i < N; ++i) {
     nl.NN[i] = (int)tmp[i].size();
@@ -315,7 +315,7 @@ NeighborList build_neighbors(const TrainSet& ts, float cutoff)
   nl.NL.resize(total);
   for (int i = 0; i < N; ++i) std::copy(tmp[i].begin(), tmp[i].end(), nl.NL.begin() + nl.start[i]);
   nl.rev.resize(total);
-  for (int i = 0; i < N; ++i)
+  parallel_for(N, [&](int i) {
     for (int s = 0; s < nl.NN[i]; ++s) {
       const int j = nl.NL[nl.start[i] + s];
       int off = 0;
@@ -326,6 +326,7 @@ NeighborList build_neighbors(const TrainSet& ts, float cutoff)
         }
       nl.rev[nl.start[i] + s] = nl.start[j] + off;
     }
+  });
   return nl;
 }
 #pragma clang fp contract(on)
@@ -336,7 +337,7 @@ Geometry build_geometry(const TrainSet& ts, const NeighborList& nl, bool triplet
   const int N = ts.N;
   g.bond.resize(nl.total);
   std::vector<double> rx(nl.total), ry(nl.total), rz(nl.total), rd(nl.total);
-  for (int c = 0; c < ts.Nc; ++c) {
+  parallel_for(ts.Nc, [&](int c) {
     const float* h = &ts.h[18 * (size_t)c];
     const int b = ts.Na_sum[c], e = b + ts.Na[c];
     for (int i = b; i < e; ++i)
@@ -359,7 +360,7 @@ Geometry build_geometry(const TrainSet& ts, const NeighborList& nl, bool triplet
         rx[k] = x; ry[k] = y; rz[k] = z; rd[k] = d;
         g.bond[k] = Vec4{(float)x, (float)y, (float)z, (float)d};
       }
-  }
+  });
   if (!triplets) return g;
   g.tri_start.resize(N);
   long long total = 0;
@@ -370,7 +371,7 @@ Geometry build_geometry(const TrainSet& ts, const NeighborList& nl, bool triplet
   if (total > 0x7fffffffLL) fail("triplet table too large");
   g.tri_total = total;
   g.tri.assign(total, Vec4{0, 0, 0, 0});
-  for (int i = 0; i < N; ++i) {
+  parallel_for(N, [&](int i) {
     const int nn = nl.NN[i], b0 = nl.start[i];
     for (int s = 0; s < nn; ++s)
       for (int t = 0; t < nn; ++t) {
@@ -385,7 +386,7 @@ Geometry build_geometry(const TrainSet& ts, const NeighborList& nl, bool triplet
           Vec4{(float)cs, (float)(rx[kt] * inv - rx[ks] * cd), (float)(ry[kt] * inv - ry[ks] * cd),
                (float)(rz[kt] * inv - rz[ks] * cd)};
       }
-  }
+  });
   return g;
 }
 

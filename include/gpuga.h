@@ -32,6 +32,7 @@ typedef struct gpuga_options {
                         0 = as many as the occupancy calculator allows */
   int pack_lanes;    /* 1: a last lane block of <= 32 individuals runs several
                         atoms per wave (three-body potentials); 0: one atom */
+  int queues;        /* work queues of the population launch; 0 = one per XCD */
 } gpuga_options;
 
 /* Thread-local text of the last failure (src/error.cu:21-25 prints it). */
@@ -108,9 +109,10 @@ int gpuga_fitness_set_timing(gpuga_fitness* h, int on);
 int gpuga_fitness_timing(gpuga_fitness* h, double* kernel_ms, long* launches, int reset);
 
 /* One population generation with its work-item timeline recorded (no
- * reference counterpart; a profiling aid).  rows[n_items][9] = configuration,
- * lane block, chunk, first atom, end atom, phase, wave slot, start ns, end ns
- * (relative to the first start).  Call with rows = NULL to get *n_items. */
+ * reference counterpart; a profiling aid).  rows[n_items][10] = configuration,
+ * lane block, chunk, first atom, end atom, phase, wave slot, start ns, end ns,
+ * and for a wave's first item the wave's entry ns (else NaN); times relative
+ * to the first start.  Call with rows = NULL to get *n_items. */
 int gpuga_fitness_trace(gpuga_fitness* h, int P, const float* genes, double* rows, int* n_items);
 
 /* Device bytes held by the handle (training set, tables, plans). */
