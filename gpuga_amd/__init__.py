@@ -86,6 +86,8 @@ def lib():
             "gpuga_ga_get_population": (C.c_int, [vp, fp]),
             "gpuga_ga_set_population": (C.c_int, [vp, fp]),
             "gpuga_ga_step": (C.c_int, [vp, C.c_int, fp, fp]),
+            "gpuga_ga_step_device": (C.c_int, [vp, C.c_int, fp, fp]),
+            "gpuga_fitness_run_ga": (C.c_int, [vp, vp, C.c_int, C.c_int, fp, fp]),
             "gpuga_run": (C.c_int, [C.c_char_p, P(Options), dp]),
             "gpuga_read_ga_in": (C.c_int, [C.c_char_p, ip, ip, ip, fp, P(C.c_uint)]),
             "gpuga_server_create": (C.c_int, [C.c_double, C.c_int, P(Options), P(vp)]),
@@ -217,6 +219,15 @@ class Fitness:
         _check(lib().gpuga_fitness_autotune(self._h, int(P), C.byref(c)))
         return c.value
 
+    def run_ga(self, ga, first_generation, generations):
+        """GA generations with population and GA on the device; returns the
+        per-generation best fitness [G] and elite genes [G][V]."""
+        best = np.zeros(generations, np.float32)
+        elite = np.zeros((generations, self.V), np.float32)
+        _check(lib().gpuga_fitness_run_ga(self._h, ga._h, int(first_generation), int(generations),
+                                          _ptr(best), _ptr(elite)))
+        return best, elite
+
     def set_timing(self, on=True):
         _check(lib().gpuga_fitness_set_timing(self._h, int(bool(on))))
 
@@ -279,6 +290,13 @@ class GA:
         f = _f32(fitness)
         out = np.zeros(self.P, np.float32)
         _check(lib().gpuga_ga_step(self._h, generation, _ptr(f), _ptr(out)))
+        return out
+
+    def step_device(self, generation, fitness):
+        """The same step by the device GA kernel (identical results)."""
+        f = _f32(fitness)
+        out = np.zeros(self.P, np.float32)
+        _check(lib().gpuga_ga_step_device(self._h, generation, _ptr(f), _ptr(out)))
         return out
 
     def __del__(self):

@@ -17,6 +17,15 @@ struct gpuga_fitness {
 };
 struct gpuga_ga {
   std::unique_ptr<GA> g;
+  // device step (gpuga_ga_step_device): its stream, state and fitness buffer
+  hipStream_t st = nullptr;
+  std::unique_ptr<DeviceGA> dev;
+  DeviceBuffer<float> fit;
+  ~gpuga_ga()
+  {
+    dev.reset();
+    if (st) (void)hipStreamDestroy(st);
+  }
 };
 struct gpuga_server {
   std::unique_ptr<Server> s;
@@ -376,6 +385,33 @@ int gpuga_ga_step(gpuga_ga* g, int generation, const float* fitness, float* sort
     if (sorted) std::memcpy(sorted, f.data(), sizeof(float) * f.size());
     g->g->crossover();
     g->g->mutation(generation);
+  });
+}
+
+int gpuga_ga_step_device(gpuga_ga* g, int generation, const float* fitness, float* sorted)
+{
+  return guarded([&] {
+    need(g, "handle");
+    need(fitness, "fitness");
+    GA& ga = *g->g;
+    if (!g->st) GPUGA_HIP(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
+    if (!g->dev) g->dev = std::make_unique<DeviceGA>(ga, nullptr, 1, g->st);
+    g->dev->upload(ga, generation, g->st);
+    g->fit.upload(fitness, (size_t)ga.params().population_size, g->st);
+    g->dev->step(g->fit.get(), g->st);
+    g->dev->download(ga, g->st);
+    if (sorted) std::memcpy(sorted, ga.fitness().data(), sizeof(float) * ga.fitness().size());
+  });
+}
+
+int gpuga_fitness_run_ga(gpuga_fitness* h, gpuga_ga* g, int first_generation, int generations,
+                         float* best, float* elite)
+{
+  return guarded([&] {
+    need(h, "handle");
+    need(g, "ga");
+    if (generations < 0) fail("generations must be >= 0");
+    h->f->run_ga(*g->g, first_generation, generations, best, elite);
   });
 }
 

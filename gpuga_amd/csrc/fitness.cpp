@@ -708,6 +708,16 @@ void Fitness::compute_async(int P, const float* genes)
   GPUGA_HIP(hipMemcpyAsync(pl.genes.get(), pl.h_genes.get(), sizeof(float) * (size_t)P * spec_.V,
                            hipMemcpyHostToDevice, st));
   if (timing_) GPUGA_HIP(hipEventRecord(ev0_, st));
+  launch_generation(pl, st);
+  if (timing_) GPUGA_HIP(hipEventRecord(ev1_, st));
+  GPUGA_HIP(hipMemcpyAsync(pl.h_out.get(), pl.out.get(),
+                           sizeof(double) * 3 * P + sizeof(float) * P + sizeof(unsigned),
+                           hipMemcpyDeviceToHost, st));
+  pending_P_ = P;
+}
+
+void Fitness::launch_generation(PopPlan& pl, hipStream_t st)
+{
   if (opt_.use_graph) {
     if (!pl.exec) {
       GPUGA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -719,11 +729,48 @@ void Fitness::compute_async(int P, const float* genes)
   } else {
     launch_population(pl, st);
   }
+}
+
+void Fitness::run_ga(GA& ga, int first_generation, int generations, float* best, float* elite)
+{
+  // GA::compute (src/ga.cu:73-92) for `generations` generations with the
+  // population on the device: each generation is the population launch on
+  // the device population followed by the device GA step; nothing returns
+  // to the host until the end
+  const int P = ga.params().population_size, V = spec_.V;
+  if (ga.V() != V) fail("the GA's parameter count differs from the potential's");
+  if (generations < 1) return;
+  PopPlan& pl = plan(P);
+  hipStream_t st = stream_;
+  DeviceGA dga(ga, pl.genes.get(), generations, st);
+  dga.upload(ga, first_generation, st);
+  if (timing_) GPUGA_HIP(hipEventRecord(ev0_, st));
+  for (int g = 0; g < generations; ++g) {
+    launch_generation(pl, st);
+    dga.step(pl.ctl.fitness, st);
+  }
   if (timing_) GPUGA_HIP(hipEventRecord(ev1_, st));
-  GPUGA_HIP(hipMemcpyAsync(pl.h_out.get(), pl.out.get(),
-                           sizeof(double) * 3 * P + sizeof(float) * P + sizeof(unsigned),
-                           hipMemcpyDeviceToHost, st));
-  pending_P_ = P;
+  unsigned status = 0;
+  GPUGA_HIP(hipMemcpyAsync(&status, pl.ctl.status, sizeof status, hipMemcpyDeviceToHost, st));
+  GPUGA_HIP(hipStreamSynchronize(st));
+  if (timing_) {
+    float ms = 0.0f;
+    GPUGA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    kernel_ms_ += ms;
+    kernel_launches_ += generations;
+  }
+  if (status != 0u) {
+    GPUGA_HIP(hipMemsetAsync(pl.ctl.status, 0, sizeof(unsigned), st));
+    GPUGA_HIP(hipStreamSynchronize(st));
+    fail("population kernel: force-assembly wait timed out; results discarded");
+  }
+  std::vector<float> h((size_t)generations * (1 + V));
+  dga.history(h.data(), generations, st);
+  for (int g = 0; g < generations; ++g) {
+    if (best) best[g] = h[(size_t)g * (1 + V)];
+    if (elite) std::memcpy(elite + (size_t)g * V, &h[(size_t)g * (1 + V) + 1], sizeof(float) * V);
+  }
+  dga.download(ga, st);
 }
 
 void Fitness::compute_wait(float* fitness, double* terms)

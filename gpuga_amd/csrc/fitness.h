@@ -8,6 +8,8 @@
 #include <vector>
 
 #include "common.h"
+#include "ga.h"
+#include "ga_device.h"
 #include "kernels.h"
 #include "trainset.h"
 
@@ -88,6 +90,11 @@ public:
   void compute_wait(float* fitness, double* terms);
   // Tune the work-item size for population size P.
   int autotune(int P);
+  // `generations` GA generations (fitness, sort, crossover, mutation) with
+  // the population and the GA on the device, from generation
+  // first_generation; ga holds the state before and after.  best[g] and
+  // elite[g][V] (genes in [0, 1]) receive each generation's ga.out record.
+  void run_ga(GA& ga, int first_generation, int generations, float* best, float* elite);
   // One generation with the item timeline recorded: rows of 10 values per
   // work item (config, lane block, chunk, atom begin, atom end, phase, wave
   // slot, start ns, end ns, entry ns of the wave for its first item else
@@ -119,6 +126,7 @@ private:
   PopPlan& plan(int P);
   void build_plan(PopPlan& pl, int P, int chunk_cost);
   void launch_population(PopPlan& pl, hipStream_t st);
+  void launch_generation(PopPlan& pl, hipStream_t st); // graph replay or direct
   double time_plan(int P, int chunk_cost, int reps);
   FitnessNorm norm() const;
   NN2BGrid nn2b_grid_desc() const;

@@ -286,3 +286,95 @@ void GA::run(const std::string& dir, const std::function<void(int, const float*,
 }
 
 } // namespace gpuga
+
+// ------------------------------------------------------------ device GA
+
+#include "ga_device.h"
+
+namespace gpuga {
+
+DeviceGA::DeviceGA(const GA& ga, float* genes, int history_cap, hipStream_t st)
+{
+  const GAParams& p = ga.params();
+  const int P = p.population_size, V = ga.V();
+  if (P > kDeviceGaMaxP)
+    fail(strfmt("the device GA supports populations of at most %d", kDeviceGaMaxP));
+  const size_t PV = (size_t)P * V;
+  if (!genes) {
+    genes_own_.resize(PV);
+    genes = genes_own_.get();
+  }
+  sorted_.resize(P);
+  copy_.resize(PV);
+  cum_.upload(ga.cumulative(), st);
+  hist_.resize((size_t)std::max(history_cap, 1) * (1 + V));
+  mt_.resize(624);
+  idx_.resize(1);
+  gen_.resize(1);
+  last_.resize(PV);
+  GPUGA_HIP(hipMemsetAsync(last_.get(), 0xff, last_.bytes(), st)); // -1
+  d_.P = P;
+  d_.V = V;
+  d_.parents = p.parent_number;
+  d_.children = P - p.parent_number;
+  d_.max_gen = p.maximum_generation;
+  d_.mutation_rate = p.mutation_rate;
+  d_.genes = genes;
+  d_.fitness = nullptr;
+  d_.sorted = sorted_.get();
+  d_.genes_copy = copy_.get();
+  d_.cumulative = cum_.get();
+  d_.mt = mt_.get();
+  d_.mt_idx = idx_.get();
+  d_.generation = gen_.get();
+  d_.history = hist_.get();
+  d_.history_cap = std::max(history_cap, 0);
+  d_.history_gen0 = 0;
+  d_.last = last_.get();
+}
+
+void DeviceGA::upload(const GA& ga, int generation, hipStream_t st)
+{
+  const size_t PV = (size_t)d_.P * d_.V;
+  GPUGA_HIP(hipMemcpyAsync(d_.genes, ga.population().data(), sizeof(float) * PV,
+                           hipMemcpyHostToDevice, st));
+  GPUGA_HIP(hipMemcpyAsync(d_.mt, ga.rng().state(), sizeof(uint32_t) * 624,
+                           hipMemcpyHostToDevice, st));
+  const int idx = ga.rng().index();
+  GPUGA_HIP(hipMemcpyAsync(d_.mt_idx, &idx, sizeof(int), hipMemcpyHostToDevice, st));
+  GPUGA_HIP(hipMemcpyAsync(d_.generation, &generation, sizeof(int), hipMemcpyHostToDevice, st));
+  GPUGA_HIP(hipStreamSynchronize(st)); // the host values are stack/host memory
+  d_.history_gen0 = generation;
+}
+
+void DeviceGA::download(GA& ga, hipStream_t st) const
+{
+  const size_t PV = (size_t)d_.P * d_.V;
+  uint32_t s[624];
+  int idx = 0;
+  GPUGA_HIP(hipMemcpyAsync(ga.population().data(), d_.genes, sizeof(float) * PV,
+                           hipMemcpyDeviceToHost, st));
+  GPUGA_HIP(hipMemcpyAsync(ga.fitness().data(), d_.sorted, sizeof(float) * d_.P,
+                           hipMemcpyDeviceToHost, st));
+  GPUGA_HIP(hipMemcpyAsync(s, d_.mt, sizeof s, hipMemcpyDeviceToHost, st));
+  GPUGA_HIP(hipMemcpyAsync(&idx, d_.mt_idx, sizeof idx, hipMemcpyDeviceToHost, st));
+  GPUGA_HIP(hipStreamSynchronize(st));
+  ga.rng().set_state(s, idx);
+}
+
+void DeviceGA::step(const float* fitness, hipStream_t st)
+{
+  GaDev d = d_;
+  d.fitness = fitness;
+  launch_ga_generation(d, st);
+}
+
+void DeviceGA::history(float* out, int n, hipStream_t st) const
+{
+  if (n > d_.history_cap) fail("history beyond its capacity");
+  GPUGA_HIP(hipMemcpyAsync(out, d_.history, sizeof(float) * (size_t)n * (1 + d_.V),
+                           hipMemcpyDeviceToHost, st));
+  GPUGA_HIP(hipStreamSynchronize(st));
+}
+
+} // namespace gpuga

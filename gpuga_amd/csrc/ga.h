@@ -67,7 +67,9 @@ public:
   std::vector<float>& population() { return population_; }
   std::vector<float>& fitness() { return fitness_; }
   const std::vector<float>& population() const { return population_; }
+  const std::vector<float>& cumulative() const { return cumulative_; }
   MT19937& rng() { return rng_; }
+  const MT19937& rng() const { return rng_; }
   int V() const { return V_; }
   const GAParams& params() const { return p_; }
 

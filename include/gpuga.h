@@ -131,6 +131,23 @@ int gpuga_ga_set_population(gpuga_ga* g, const float* genes);
  * mutation(generation) (src/ga.cu:83-88).  sorted_fitness[P] may be NULL. */
 int gpuga_ga_step(gpuga_ga* g, int generation, const float* fitness, float* sorted_fitness);
 
+/* The same step computed by the device GA kernel, with identical results
+ * (no reference counterpart; it is what gpuga_fitness_run_ga runs each
+ * generation, exposed so that it can be checked against gpuga_ga_step).
+ * Populations of at most 4096. */
+int gpuga_ga_step_device(gpuga_ga* g, int generation, const float* fitness, float* sorted_fitness);
+
+/* GA::compute, src/ga.cu:73-92: `generations` generations (Fitness::compute,
+ * sort_population, crossover, mutation) starting at generation
+ * first_generation, with the population and the GA step on the device and
+ * no host round trip in between; the GA handle holds the state before and
+ * after (bit-identical to gpuga_fitness_compute + gpuga_ga_step per
+ * generation).  best[generations] and elite[generations][V] (genes in [0, 1])
+ * receive the ga.out record of each generation (src/ga.cu:138-159); either
+ * may be NULL.  Populations of at most 4096. */
+int gpuga_fitness_run_ga(gpuga_fitness* h, gpuga_ga* g, int first_generation, int generations,
+                         float* best, float* elite);
+
 /* ga.in reader of GA::read_parameters (src/ga.cu:214-316); host only.  Also
  * accepts an optional `seed` keyword (default 12345678, the reference's
  * deterministic seed, src/ga.cu:47-48). */

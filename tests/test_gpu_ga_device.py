@@ -1,0 +1,77 @@
+"""Device GA step (ga_device.hip) against the host GA (ga.cpp, itself pinned
+bit-exact to the oracle and libstdc++ <random>): the same populations, sorted
+fitness and random state, generation after generation.
+
+Shapes cover the selection table, crossover and mutation paths: with
+P = 256, V = 1153 (the NN2B n = 32 layout) a gene-index draw is rejected
+with probability 290191 / 2^32 = 6.8e-5, about 4 times in each early
+generation's 59034 mutations, so the redraw path between the parallel
+mutation batches runs many times."""
+import numpy as np
+import pytest
+
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+
+def fitness_seq(P, gens, seed, ties=False, nan_at=None):
+    rng = np.random.default_rng(seed)
+    out = []
+    for g in range(gens):
+        f = rng.random(P).astype(np.float32)
+        if ties:
+            f = np.round(f * 8) / 8  # many equal values: stability of the sort
+        if nan_at is not None and g == nan_at:
+            f[rng.integers(0, P, 3)] = np.nan
+        out.append(f.astype(np.float32))
+    return out
+
+
+@pytest.mark.parametrize("P,parents,V,gens,ties,nan_at", [
+    (20, 10, 9, 40, False, None),
+    (200, 100, 9, 30, True, None),
+    (64, 32, 141, 20, False, 3),
+    (130, 64, 15, 20, True, 5),
+    (256, 128, 1153, 6, False, None),
+])
+def test_device_step_equals_host_step(P, parents, V, gens, ties, nan_at):
+    import gpuga_amd as G
+    print_case("device GA step", P, parents, V)
+    host = G.GA(P, parents, V, maximum_generation=gens, mutation_rate=0.2, seed=777)
+    dev = G.GA(P, parents, V, maximum_generation=gens, mutation_rate=0.2, seed=777)
+    assert np.array_equal(host.population, dev.population)
+    for g, f in enumerate(fitness_seq(P, gens, 31 + P, ties, nan_at)):
+        sh = host.step(g, f)
+        sd = dev.step_device(g, f)
+        assert np.array_equal(sh, sd, equal_nan=True), g
+        assert np.array_equal(host.population, dev.population), g
+
+
+@pytest.mark.parametrize("name,P,parents", [("si_diamond", 64, 32), ("hfo2", 40, 20),
+                                            ("gaas", 200, 100)])
+def test_run_ga_equals_host_loop(golden, name, P, parents):
+    """gpuga_fitness_run_ga (population kernel + device GA step per
+    generation, no host round trip) against compute + host GA step."""
+    import gpuga_amd as G
+    print_case("run_ga", name, P)
+    fit = G.Fitness(golden(name))
+    gens = 12
+    host = G.GA(P, parents, fit.V, maximum_generation=gens, mutation_rate=0.2, seed=4242)
+    dev = G.GA(P, parents, fit.V, maximum_generation=gens, mutation_rate=0.2, seed=4242)
+    best_h, elite_h = [], []
+    for g in range(gens):
+        f = fit.compute(host.population)
+        s = host.step(g, f)
+        best_h.append(s[0])
+        elite_h.append(host.population[0].copy())  # row 0 is never crossed or mutated
+    best_d, elite_d = fit.run_ga(dev, 0, gens)
+    assert np.array_equal(np.array(best_h, np.float32), best_d)
+    assert np.array_equal(np.array(elite_h), elite_d)
+    assert np.array_equal(host.population, dev.population)
+    # continuing from the returned state matches the host continuing too
+    f = fit.compute(host.population)
+    s = host.step(gens, f)
+    b2, e2 = fit.run_ga(dev, gens, 1)
+    assert b2[0] == s[0]
+    assert np.array_equal(host.population, dev.population)
