@@ -125,6 +125,8 @@ size_t DeviceSet::bytes() const
 
 PopPlan::~PopPlan()
 {
+  if (gen_exec) (void)hipGraphExecDestroy(gen_exec);
+  if (gen_graph) (void)hipGraphDestroy(gen_graph);
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
 }
@@ -742,33 +744,56 @@ void Fitness::run_ga(GA& ga, int first_generation, int generations, float* best,
   if (generations < 1) return;
   PopPlan& pl = plan(P);
   hipStream_t st = stream_;
-  DeviceGA dga(ga, pl.genes.get(), generations, st);
+  if (!pl.dga || !pl.dga->matches(ga)) {
+    if (pl.gen_exec) GPUGA_HIP(hipGraphExecDestroy(pl.gen_exec));
+    if (pl.gen_graph) GPUGA_HIP(hipGraphDestroy(pl.gen_graph));
+    pl.gen_exec = nullptr;
+    pl.gen_graph = nullptr;
+    pl.dga = std::make_unique<DeviceGA>(ga, pl.genes.get(), 1024, st);
+  }
+  DeviceGA& dga = *pl.dga;
   dga.upload(ga, first_generation, st);
-  if (timing_) GPUGA_HIP(hipEventRecord(ev0_, st));
-  for (int g = 0; g < generations; ++g) {
-    launch_generation(pl, st);
+  if (opt_.use_graph && !pl.gen_exec) {
+    GPUGA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    launch_population(pl, st);
     dga.step(pl.ctl.fitness, st);
+    GPUGA_HIP(hipStreamEndCapture(st, &pl.gen_graph));
+    GPUGA_HIP(hipGraphInstantiate(&pl.gen_exec, pl.gen_graph, nullptr, nullptr, 0));
   }
-  if (timing_) GPUGA_HIP(hipEventRecord(ev1_, st));
-  unsigned status = 0;
-  GPUGA_HIP(hipMemcpyAsync(&status, pl.ctl.status, sizeof status, hipMemcpyDeviceToHost, st));
-  GPUGA_HIP(hipStreamSynchronize(st));
-  if (timing_) {
-    float ms = 0.0f;
-    GPUGA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
-    kernel_ms_ += ms;
-    kernel_launches_ += generations;
-  }
-  if (status != 0u) {
-    GPUGA_HIP(hipMemsetAsync(pl.ctl.status, 0, sizeof(unsigned), st));
+  std::vector<float> h((size_t)dga.history_cap() * (1 + V));
+  for (int g0 = 0; g0 < generations; g0 += dga.history_cap()) {
+    const int n = std::min(dga.history_cap(), generations - g0);
+    if (g0 > 0) dga.restart_history(first_generation + g0, st);
+    if (timing_) GPUGA_HIP(hipEventRecord(ev0_, st));
+    for (int g = 0; g < n; ++g) {
+      if (pl.gen_exec) {
+        GPUGA_HIP(hipGraphLaunch(pl.gen_exec, st));
+      } else {
+        launch_population(pl, st);
+        dga.step(pl.ctl.fitness, st);
+      }
+    }
+    if (timing_) GPUGA_HIP(hipEventRecord(ev1_, st));
+    unsigned status = 0;
+    GPUGA_HIP(hipMemcpyAsync(&status, pl.ctl.status, sizeof status, hipMemcpyDeviceToHost, st));
     GPUGA_HIP(hipStreamSynchronize(st));
-    fail("population kernel: force-assembly wait timed out; results discarded");
-  }
-  std::vector<float> h((size_t)generations * (1 + V));
-  dga.history(h.data(), generations, st);
-  for (int g = 0; g < generations; ++g) {
-    if (best) best[g] = h[(size_t)g * (1 + V)];
-    if (elite) std::memcpy(elite + (size_t)g * V, &h[(size_t)g * (1 + V) + 1], sizeof(float) * V);
+    if (timing_) {
+      float ms = 0.0f;
+      GPUGA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+      kernel_ms_ += ms;
+      kernel_launches_ += n;
+    }
+    if (status != 0u) {
+      GPUGA_HIP(hipMemsetAsync(pl.ctl.status, 0, sizeof(unsigned), st));
+      GPUGA_HIP(hipStreamSynchronize(st));
+      fail("population kernel: force-assembly wait timed out; results discarded");
+    }
+    dga.history(h.data(), n, st);
+    for (int g = 0; g < n; ++g) {
+      if (best) best[g0 + g] = h[(size_t)g * (1 + V)];
+      if (elite)
+        std::memcpy(elite + (size_t)(g0 + g) * V, &h[(size_t)g * (1 + V) + 1], sizeof(float) * V);
+    }
   }
   dga.download(ga, st);
 }
@@ -982,7 +1007,9 @@ double Fitness::time_plan(int P, int chunk_cost, int reps)
 int Fitness::autotune(int P)
 {
   // Candidate work-item sizes: the whole-configuration cost spread over
-  // 1/2 .. 1/32 of the resident waves.
+  // 1/2 .. 1/32 of the resident waves, and the largest configuration's cost
+  // halved step by step (small sets, where latency rather than balance
+  // sets the time).
   long long total = 0, maxcfg = 0;
   for (int c = 0; c < ts_.Nc; ++c) {
     long long t = 0;
@@ -1001,6 +1028,7 @@ int Fitness::autotune(int P)
     cands.push_back((int)c);
   }
   cands.push_back((int)std::max<long long>(maxcfg, 64)); // whole configurations
+  for (long long c = maxcfg / 2; c >= 64; c /= 2) cands.push_back((int)c);
   std::sort(cands.begin(), cands.end());
   cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
   int best = cands.back();

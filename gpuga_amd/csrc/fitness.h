@@ -68,6 +68,11 @@ struct PopPlan {
   PopCtl ctl{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // device GA of this population size and one whole generation (population
+  // launch + GA step) as a graph
+  std::unique_ptr<DeviceGA> dga;
+  hipGraph_t gen_graph = nullptr;
+  hipGraphExec_t gen_exec = nullptr;
   ~PopPlan();
 };
 

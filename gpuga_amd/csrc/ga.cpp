@@ -313,6 +313,10 @@ DeviceGA::DeviceGA(const GA& ga, float* genes, int history_cap, hipStream_t st)
   gen_.resize(1);
   last_.resize(PV);
   GPUGA_HIP(hipMemsetAsync(last_.get(), 0xff, last_.bytes(), st)); // -1
+  dec_.resize(3 * (size_t)std::max((P - p.parent_number + 1) / 2, 1));
+  mut_idx_.resize(PV);
+  mut_val_.resize(PV);
+  mut_count_.resize(1);
   d_.P = P;
   d_.V = V;
   d_.parents = p.parent_number;
@@ -329,8 +333,20 @@ DeviceGA::DeviceGA(const GA& ga, float* genes, int history_cap, hipStream_t st)
   d_.generation = gen_.get();
   d_.history = hist_.get();
   d_.history_cap = std::max(history_cap, 0);
-  d_.history_gen0 = 0;
+  hist_gen0_.resize(1);
+  d_.history_gen0 = hist_gen0_.get();
   d_.last = last_.get();
+  d_.dec = dec_.get();
+  d_.mut_idx = mut_idx_.get();
+  d_.mut_val = mut_val_.get();
+  d_.mut_count = mut_count_.get();
+}
+
+bool DeviceGA::matches(const GA& ga) const
+{
+  const GAParams& p = ga.params();
+  return d_.P == p.population_size && d_.V == ga.V() && d_.parents == p.parent_number &&
+         d_.max_gen == p.maximum_generation && d_.mutation_rate == p.mutation_rate;
 }
 
 void DeviceGA::upload(const GA& ga, int generation, hipStream_t st)
@@ -343,8 +359,14 @@ void DeviceGA::upload(const GA& ga, int generation, hipStream_t st)
   const int idx = ga.rng().index();
   GPUGA_HIP(hipMemcpyAsync(d_.mt_idx, &idx, sizeof(int), hipMemcpyHostToDevice, st));
   GPUGA_HIP(hipMemcpyAsync(d_.generation, &generation, sizeof(int), hipMemcpyHostToDevice, st));
+  GPUGA_HIP(hipMemcpyAsync(d_.history_gen0, &generation, sizeof(int), hipMemcpyHostToDevice, st));
   GPUGA_HIP(hipStreamSynchronize(st)); // the host values are stack/host memory
-  d_.history_gen0 = generation;
+}
+
+void DeviceGA::restart_history(int generation, hipStream_t st)
+{
+  GPUGA_HIP(hipMemcpyAsync(d_.history_gen0, &generation, sizeof(int), hipMemcpyHostToDevice, st));
+  GPUGA_HIP(hipStreamSynchronize(st));
 }
 
 void DeviceGA::download(GA& ga, hipStream_t st) const

@@ -24,9 +24,13 @@ struct GaDev {
   int* mt_idx;             // [1] position in the state block
   int* generation;         // [1] generation number, advanced by each step
   float* history;          // [history_cap][1 + V] best fitness and elite row
-  int history_cap;         // rows; generation g goes to row g - history_gen0
-  int history_gen0;
+  int history_cap;         // rows; generation g goes to row g - *history_gen0
+  int* history_gen0;       // [1]
   int* last;               // [P][V] scratch, -1 between steps
+  int* dec;                // [children / 2][3] crossover parents and point
+  int* mut_idx;            // [P * V] mutation list: gene index ...
+  float* mut_val;          // [P * V] ... and value, in draw order
+  int* mut_count;          // [1]
 };
 
 // Largest population the one-workgroup step supports (LDS budget).
@@ -41,8 +45,14 @@ void launch_ga_generation(const GaDev& d, hipStream_t st);
 class DeviceGA {
 public:
   DeviceGA(const GA& ga, float* genes, int history_cap, hipStream_t st);
-  // host GA -> device (population, random state, generation number)
+  // whether this state was built for a GA of ga's shape and settings
+  bool matches(const GA& ga) const;
+  // host GA -> device (population, random state, generation number); the
+  // history restarts at that generation
   void upload(const GA& ga, int generation, hipStream_t st);
+  // the history restarts at `generation` (for runs longer than its capacity)
+  void restart_history(int generation, hipStream_t st);
+  int history_cap() const { return d_.history_cap; }
   // device -> host GA (population, random state, sorted fitness)
   void download(GA& ga, hipStream_t st) const;
   // one generation step on fitness (device pointer)
@@ -54,9 +64,9 @@ public:
 
 private:
   GaDev d_{};
-  DeviceBuffer<float> genes_own_, sorted_, copy_, cum_, hist_, fit_;
+  DeviceBuffer<float> genes_own_, sorted_, copy_, cum_, hist_, fit_, mut_val_;
   DeviceBuffer<uint32_t> mt_;
-  DeviceBuffer<int> idx_, gen_, last_;
+  DeviceBuffer<int> idx_, gen_, last_, dec_, mut_idx_, mut_count_, hist_gen0_;
 };
 
 } // namespace gpuga

@@ -84,7 +84,6 @@ __global__ void __launch_bounds__(256) nn2b_tabulate_f32(PopCtl ctl, NN2BGrid gr
 struct NN2BPop {
   static constexpr bool kForcePass = false;
   static constexpr bool kPacked = false;
-  static constexpr int kMinWaves = 1;
   struct Lane {
     int dummy;
   };

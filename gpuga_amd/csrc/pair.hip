@@ -138,7 +138,6 @@ __device__ __forceinline__ void ri_pair_pop(const RILane& q, float d, const floa
 struct RIPop {
   static constexpr bool kForcePass = false;
   static constexpr bool kPacked = false;
-  static constexpr int kMinWaves = 1;
   using Lane = RILane;
   using Extra = const float4*;
 

@@ -299,12 +299,13 @@ __device__ inline double group_sum(double x, int W)
 //   static double force_pass_packed(set, ctl, a0, a1, k, col, g, G)
 //                                                   the same for lane group g
 // PACK selects the instantiation that also serves packed items (chosen per
-// item); the other one never contains the packed code.  Pot::kMinWaves is
-// the occupancy the register allocator must keep (waves per SIMD), so the
-// packed code cannot cost the common path a wave.
+// item); the other one never contains the packed code, so its register
+// budget is unaffected.  (Forcing the PACK one to the other's occupancy
+// makes it spill to scratch, and a launch that needs scratch memory costs
+// more than the wave it saves.)
 template <class Pot, bool PACK = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Pot::kMinWaves)))
-population_kernel(PopCtl ctl, DevSet set, typename Pot::Extra ex)
+__global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
+                                                         typename Pot::Extra ex)
 {
   const int lane = __lane_id();
   const unsigned total_waves = gridDim.x * (blockDim.x / WAVE);

@@ -291,9 +291,12 @@ struct TersoffPop {
         }
         float b, bp;
         tersoff_bond_order(q, zeta, b, bp);
-        const int k = TWO ? ((unl >> u) & 1) : 0;
+        // the parameter set as a branch on the (wave-uniform) bond type, so
+        // the set is a compile-time index and the parameters stay in
+        // registers (a run-time index puts them in scratch memory)
         float fr, frp, fa, fap;
-        tersoff_pair(q, k, dist(u), fr, frp, fa, fap);
+        if (TWO && ((unl >> u) & 1)) tersoff_pair(q, 1, dist(u), fr, frp, fa, fap);
+        else tersoff_pair(q, 0, dist(u), fr, frp, fa, fap);
         const float rep = fr - b * fa;
         e = fmaf(fc[u], rep, e);
         const float di = dist_inv(u);
@@ -391,9 +394,6 @@ struct TersoffPop {
   // Packed item: lane group g takes the atoms a0 + g, a0 + g + G, ...; the
   // per-bond forces go to lane column col (the lane's individual).
   static constexpr bool kPacked = true;
-  // five waves per SIMD at NMAX = 4 (the common path needs 92 VGPRs; the
-  // packed path must fit beside it); no constraint for the larger bounds
-  static constexpr int kMinWaves = NMAX == 4 ? 5 : 1;
   __device__ static void chunk_packed(const DevSet& s, const TLane& q, const WorkItem& it,
                                       const PopCtl& ctl, int, int col, int g, int G, Acc& acc)
   {
@@ -417,10 +417,10 @@ struct TersoffPop {
     const float* fblk = ctl.bondf + (size_t)k * ctl.B_force * 3 * WAVE + lane;
     const long N = s.N;
     double err = 0.0;
-    // four atoms per step, every load unconditional, so all of them are in
+    // two atoms per step, every load unconditional, so all of them are in
     // flight together (the loads go to memory: their latency, not their
-    // number, sets the time)
-    constexpr int STEP = 4;
+    // number, sets the time; four per step would cost the kernel a wave)
+    constexpr int STEP = 2;
     for (int i = a0; i < a1; i += STEP) {
       float d[STEP][3];
 #pragma unroll
