@@ -115,6 +115,7 @@ def serve(args):
             time.sleep(max(0.0, min(0.01, arrivals[submitted] - now)))
     wall = time.perf_counter() - t0
     lat = srv.latencies()
+    adm = srv.admissions()
     gens = len(lat)
     work = sum(P * N * args.generations for _, _, P, N in jobs)
     out = {
@@ -134,6 +135,8 @@ def serve(args):
                    "generations_per_job": args.generations, "wall_s": wall,
                    "p50_generation_ms": float(np.percentile(lat, 50)),
                    "p99_generation_ms": float(np.percentile(lat, 99)),
+                   "p50_admission_ms": float(np.percentile(adm, 50)),
+                   "p99_admission_ms": float(np.percentile(adm, 99)),
                    "individual_atom_evals_per_s": work / wall,
                    "mix": [f"{k}:P{P}:N{N}" for _, k, P, N in jobs]},
     }

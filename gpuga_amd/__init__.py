@@ -97,6 +97,7 @@ def lib():
             "gpuga_server_poll": (C.c_int, [vp, ip, ip, ip]),
             "gpuga_server_job": (C.c_int, [vp, C.c_int, ip, ip, fp, fp]),
             "gpuga_server_latencies": (C.c_int, [vp, dp, ip]),
+            "gpuga_server_admissions": (C.c_int, [vp, dp, ip]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -371,10 +372,18 @@ class Server:
                 "elite": elite}
 
     def latencies(self):
+        """Per-generation latencies (ms), all jobs, completion order."""
+        return self._times(lib().gpuga_server_latencies)
+
+    def admissions(self):
+        """Per admitted job: submission to ready (ms)."""
+        return self._times(lib().gpuga_server_admissions)
+
+    def _times(self, fn):
         n = C.c_int(0)
-        _check(lib().gpuga_server_latencies(self._h, None, C.byref(n)))
+        _check(fn(self._h, None, C.byref(n)))
         out = np.zeros(max(n.value, 1))
-        _check(lib().gpuga_server_latencies(self._h, _ptr(out, C.c_double), C.byref(n)))
+        _check(fn(self._h, _ptr(out, C.c_double), C.byref(n)))
         return out[: n.value]
 
     def __del__(self):

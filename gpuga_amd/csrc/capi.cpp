@@ -513,4 +513,17 @@ int gpuga_server_latencies(const gpuga_server* s, double* ms, int* n)
   });
 }
 
+int gpuga_server_admissions(const gpuga_server* s, double* ms, int* n)
+{
+  return guarded([&] {
+    need(s, "handle");
+    need(n, "n");
+    const std::vector<double>& v = s->s->admissions();
+    const int m = std::min<int>(*n, (int)v.size());
+    if (ms)
+      for (int i = 0; i < m; ++i) ms[i] = v[i];
+    *n = (int)v.size();
+  });
+}
+
 } // extern "C"

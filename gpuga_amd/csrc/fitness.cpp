@@ -721,16 +721,27 @@ void Fitness::compute_async(int P, const float* genes)
 void Fitness::launch_generation(PopPlan& pl, hipStream_t st)
 {
   if (opt_.use_graph) {
-    if (!pl.exec) {
-      GPUGA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      launch_population(pl, st);
-      GPUGA_HIP(hipStreamEndCapture(st, &pl.graph));
-      GPUGA_HIP(hipGraphInstantiate(&pl.exec, pl.graph, nullptr, nullptr, 0));
-    }
+    capture(pl, st);
     GPUGA_HIP(hipGraphLaunch(pl.exec, st));
   } else {
     launch_population(pl, st);
   }
+}
+
+void Fitness::capture(PopPlan& pl, hipStream_t st)
+{
+  if (pl.exec) return;
+  GPUGA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  launch_population(pl, st);
+  GPUGA_HIP(hipStreamEndCapture(st, &pl.graph));
+  GPUGA_HIP(hipGraphInstantiate(&pl.exec, pl.graph, nullptr, nullptr, 0));
+}
+
+void Fitness::prepare(int P)
+{
+  if (opt_.backend == 1) return;
+  PopPlan& pl = plan(P);
+  if (opt_.use_graph) capture(pl, stream_);
 }
 
 void Fitness::run_ga(GA& ga, int first_generation, int generations, float* best, float* elite)

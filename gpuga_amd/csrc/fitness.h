@@ -95,6 +95,9 @@ public:
   void compute_wait(float* fitness, double* terms);
   // Tune the work-item size for population size P.
   int autotune(int P);
+  // Build the work plan (and its graph) for population size P now, so the
+  // first generation does not pay for it.
+  void prepare(int P);
   // `generations` GA generations (fitness, sort, crossover, mutation) with
   // the population and the GA on the device, from generation
   // first_generation; ga holds the state before and after.  best[g] and
@@ -132,6 +135,7 @@ private:
   void build_plan(PopPlan& pl, int P, int chunk_cost);
   void launch_population(PopPlan& pl, hipStream_t st);
   void launch_generation(PopPlan& pl, hipStream_t st); // graph replay or direct
+  void capture(PopPlan& pl, hipStream_t st);           // the plan's graph, once
   double time_plan(int P, int chunk_cost, int reps);
   FitnessNorm norm() const;
   NN2BGrid nn2b_grid_desc() const;

@@ -21,6 +21,7 @@ Server::~Server()
 int Server::submit(const std::string& dir, bool write_outputs)
 {
   auto j = std::make_unique<Job>();
+  j->t_submit = std::chrono::steady_clock::now();
   j->id = (int)jobs_.size();
   j->dir = dir;
   j->write = write_outputs;
@@ -59,6 +60,9 @@ void Server::admit(Job& j)
     j.fit = std::make_unique<Fitness>(j.spec, std::move(j.ts), o);
     // deterministic work-item size: the same chunking alone or co-scheduled
     if (o.chunk_cost <= 0) j.fit->options().chunk_cost = j.fit->heuristic_chunk_cost(j.gp.population_size);
+    // plan and graph at admission: generation latencies then measure
+    // generations only (admission time is kept per job)
+    j.fit->prepare(j.gp.population_size);
     j.ga = std::make_unique<GA>(j.gp, j.fit->number_of_variables(), j.fit->spec().pmin,
                                 j.fit->spec().pmax);
     if (j.write) {
@@ -68,6 +72,9 @@ void Server::admit(Job& j)
     }
     j.state = 1;
     used_ += (double)j.est_bytes;
+    adm_ms_.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                                j.t_submit)
+                        .count());
   } catch (const std::exception& e) {
     retire(j, false, e.what());
   }

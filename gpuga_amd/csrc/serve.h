@@ -34,6 +34,8 @@ public:
   int finished() const;
   void job_info(int id, int* state, int* gens, float* best, float* elite) const;
   const std::vector<double>& latencies() const { return lat_ms_; }
+  // per admitted job: submission to ready (parse, set-up, plan, graph), ms
+  const std::vector<double>& admissions() const { return adm_ms_; }
 
 private:
   struct Job {
@@ -51,7 +53,7 @@ private:
     FILE* gaout = nullptr;
     int gen = 0;
     bool inflight = false;
-    std::chrono::steady_clock::time_point t_launch;
+    std::chrono::steady_clock::time_point t_launch, t_submit;
     float best = 0.0f;
     std::vector<float> elite;
   };
@@ -66,7 +68,7 @@ private:
   std::vector<std::unique_ptr<Job>> jobs_;
   std::deque<int> queue_;
   double used_ = 0.0;
-  std::vector<double> lat_ms_;
+  std::vector<double> lat_ms_, adm_ms_;
 };
 
 } // namespace gpuga

@@ -180,6 +180,9 @@ int gpuga_server_job(const gpuga_server* s, int job_id, int* state, int* generat
                      float* best_fitness, float* elite);
 /* Per-generation latencies (ms) of all jobs in completion order; n in/out. */
 int gpuga_server_latencies(const gpuga_server* s, double* ms, int* n);
+/* Per admitted job, submission to ready (parsing, set-up, work plan, graph;
+ * queueing behind the memory budget included), ms, in admission order. */
+int gpuga_server_admissions(const gpuga_server* s, double* ms, int* n);
 
 #ifdef __cplusplus
 }
