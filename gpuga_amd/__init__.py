@@ -38,7 +38,8 @@ class GpugaError(RuntimeError):
 class Options(C.Structure):
     _fields_ = [("backend", C.c_int), ("chunk_cost", C.c_int), ("use_graph", C.c_int),
                 ("nn2b_bf16", C.c_int), ("nn2b_grid", C.c_int), ("verbose", C.c_int),
-                ("blocks_per_cu", C.c_int), ("pack_lanes", C.c_int), ("queues", C.c_int)]
+                ("blocks_per_cu", C.c_int), ("pack_lanes", C.c_int), ("queues", C.c_int),
+                ("ga_batch", C.c_int)]
 
 
 def build(jobs: int = 8) -> None:

@@ -62,6 +62,7 @@ FitnessOptions to_options(const gpuga_options* o)
   f.blocks_per_cu = o->blocks_per_cu;
   f.pack_lanes = o->pack_lanes;
   f.queues = o->queues;
+  f.ga_batch = o->ga_batch;
   return f;
 }
 
@@ -89,6 +90,7 @@ void gpuga_default_options(gpuga_options* opt)
   opt->blocks_per_cu = f.blocks_per_cu;
   opt->pack_lanes = f.pack_lanes;
   opt->queues = f.queues;
+  opt->ga_batch = f.ga_batch;
 }
 
 int gpuga_fitness_create(const char* input_dir, const gpuga_options* opt, gpuga_fitness** out)
@@ -398,8 +400,9 @@ int gpuga_ga_step_device(gpuga_ga* g, int generation, const float* fitness, floa
     if (!g->dev) g->dev = std::make_unique<DeviceGA>(ga, nullptr, 1, g->st);
     g->dev->upload(ga, generation, g->st);
     g->fit.upload(fitness, (size_t)ga.params().population_size, g->st);
-    g->dev->step(g->fit.get(), g->st);
-    g->dev->download(ga, g->st);
+    g->dev->stage(ga, generation, 1, g->st);
+    g->dev->step(0, g->fit.get(), g->st);
+    g->dev->download(ga, 1, g->st);
     if (sorted) std::memcpy(sorted, ga.fitness().data(), sizeof(float) * ga.fitness().size());
   });
 }

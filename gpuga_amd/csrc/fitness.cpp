@@ -123,10 +123,20 @@ size_t DeviceSet::bytes() const
          virial_ref.bytes() + force_ref.bytes();
 }
 
+void PopPlan::drop_ga_graphs()
+{
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < kGaGraphs; ++j) {
+      if (ga_exec[k][j]) (void)hipGraphExecDestroy(ga_exec[k][j]);
+      if (ga_graph[k][j]) (void)hipGraphDestroy(ga_graph[k][j]);
+      ga_exec[k][j] = nullptr;
+      ga_graph[k][j] = nullptr;
+    }
+}
+
 PopPlan::~PopPlan()
 {
-  if (gen_exec) (void)hipGraphExecDestroy(gen_exec);
-  if (gen_graph) (void)hipGraphDestroy(gen_graph);
+  drop_ga_graphs();
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
 }
@@ -682,19 +692,21 @@ PopPlan& Fitness::plan(int P)
   return ref;
 }
 
-void Fitness::launch_population(PopPlan& pl, hipStream_t st)
+void Fitness::launch_population(PopPlan& pl, hipStream_t st, const float* genes)
 {
+  PopCtl ctl = pl.ctl;
+  if (genes) ctl.genes = genes;
   switch (spec_.type) {
-  case POT_RI: launch_ri_population(pl.ctl, ds_.view, ds_.ri_tab.get(), pl.grid, pl.block, st); break;
+  case POT_RI: launch_ri_population(ctl, ds_.view, ds_.ri_tab.get(), pl.grid, pl.block, st); break;
   case POT_NN2B: {
     NN2BGrid g = nn2b_grid_desc();
-    launch_nn2b_tabulate(pl.ctl, g, pl.nn2b_table.get(), st);
-    launch_nn2b_population(pl.ctl, ds_.view, g, pl.nn2b_table.get(), pl.grid, pl.block, st);
+    launch_nn2b_tabulate(ctl, g, pl.nn2b_table.get(), st);
+    launch_nn2b_population(ctl, ds_.view, g, pl.nn2b_table.get(), pl.grid, pl.block, st);
     break;
   }
   default: {
     // the instantiation with the packed path only when the plan has packed items
-    launch_tersoff_population(pl.ctl, ds_.view, nn_class(nl_.max_nn), pl.grid, pl.block,
+    launch_tersoff_population(ctl, ds_.view, nn_class(nl_.max_nn), pl.grid, pl.block,
                               pl.groups_last > 1, st);
     break;
   }
@@ -744,44 +756,68 @@ void Fitness::prepare(int P)
   if (opt_.use_graph) capture(pl, stream_);
 }
 
+static_assert((1 << (PopPlan::kGaGraphs - 1)) >= kGaMaxBatch, "a batch fits the largest graph");
+
+void Fitness::capture_generations(PopPlan& pl, int k, int n, hipGraph_t* graph,
+                                  hipGraphExec_t* exec)
+{
+  hipStream_t st = stream_;
+  GPUGA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < n; ++i) {
+    const int b = (k + i) & 1;
+    launch_population(pl, st, pl.dga->genes(b));
+    pl.dga->step(b, pl.ctl.fitness, st);
+  }
+  GPUGA_HIP(hipStreamEndCapture(st, graph));
+  GPUGA_HIP(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+}
+
 void Fitness::run_ga(GA& ga, int first_generation, int generations, float* best, float* elite)
 {
   // GA::compute (src/ga.cu:73-92) for `generations` generations with the
   // population on the device: each generation is the population launch on
-  // the device population followed by the device GA step; nothing returns
-  // to the host until the end
+  // one population buffer followed by the device GA step into the other;
+  // the GA's random decisions are drawn on the host a batch ahead (they do
+  // not depend on the fitness) and nothing returns to the host until the end
   const int P = ga.params().population_size, V = spec_.V;
   if (ga.V() != V) fail("the GA's parameter count differs from the potential's");
+  if (first_generation < 0) fail("generation numbers start at 0");
   if (generations < 1) return;
   PopPlan& pl = plan(P);
   hipStream_t st = stream_;
   if (!pl.dga || !pl.dga->matches(ga)) {
-    if (pl.gen_exec) GPUGA_HIP(hipGraphExecDestroy(pl.gen_exec));
-    if (pl.gen_graph) GPUGA_HIP(hipGraphDestroy(pl.gen_graph));
-    pl.gen_exec = nullptr;
-    pl.gen_graph = nullptr;
-    pl.dga = std::make_unique<DeviceGA>(ga, pl.genes.get(), 1024, st);
+    pl.drop_ga_graphs();
+    pl.dga = std::make_unique<DeviceGA>(ga, pl.genes.get(), 1024, st, opt_.ga_batch);
   }
   DeviceGA& dga = *pl.dga;
   dga.upload(ga, first_generation, st);
-  if (opt_.use_graph && !pl.gen_exec) {
-    GPUGA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    launch_population(pl, st);
-    dga.step(pl.ctl.fitness, st);
-    GPUGA_HIP(hipStreamEndCapture(st, &pl.gen_graph));
-    GPUGA_HIP(hipGraphInstantiate(&pl.gen_exec, pl.gen_graph, nullptr, nullptr, 0));
-  }
-  std::vector<float> h((size_t)dga.history_cap() * (1 + V));
-  for (int g0 = 0; g0 < generations; g0 += dga.history_cap()) {
-    const int n = std::min(dga.history_cap(), generations - g0);
-    if (g0 > 0) dga.restart_history(first_generation + g0, st);
+  // history chunks of whole batches, so that every batch but the last is full
+  const int B = dga.batch();
+  const int H = std::max(B, dga.history_cap() / B * B);
+  std::vector<float> h((size_t)H * (1 + V));
+  int g = 0; // generations queued
+  while (g < generations) {
+    const int n = std::min(H, generations - g);
+    if (g > 0) dga.restart_history(first_generation + g, st);
     if (timing_) GPUGA_HIP(hipEventRecord(ev0_, st));
-    for (int g = 0; g < n; ++g) {
-      if (pl.gen_exec) {
-        GPUGA_HIP(hipGraphLaunch(pl.gen_exec, st));
-      } else {
-        launch_population(pl, st);
-        dga.step(pl.ctl.fitness, st);
+    for (int b = 0; b < n; b += B) {
+      const int nb = std::min(B, n - b);
+      dga.stage(ga, first_generation + g + b, nb, st);
+      // the batch as runs of 2^j generations, each one graph (no launch
+      // gap between the generations inside a graph)
+      for (int i = 0; i < nb;) {
+        const int k = (g + b + i) & 1;
+        if (!opt_.use_graph) {
+          launch_population(pl, st, dga.genes(k));
+          dga.step(k, pl.ctl.fitness, st);
+          ++i;
+          continue;
+        }
+        int j = 0;
+        while (j + 1 < PopPlan::kGaGraphs && (2 << j) <= nb - i) ++j;
+        if (!pl.ga_exec[k][j]) capture_generations(pl, k, 1 << j, &pl.ga_graph[k][j], &pl.ga_exec[k][j]);
+        GPUGA_HIP(hipGraphLaunch(pl.ga_exec[k][j], st));
+        i += 1 << j;
       }
     }
     if (timing_) GPUGA_HIP(hipEventRecord(ev1_, st));
@@ -800,13 +836,14 @@ void Fitness::run_ga(GA& ga, int first_generation, int generations, float* best,
       fail("population kernel: force-assembly wait timed out; results discarded");
     }
     dga.history(h.data(), n, st);
-    for (int g = 0; g < n; ++g) {
-      if (best) best[g0 + g] = h[(size_t)g * (1 + V)];
+    for (int i = 0; i < n; ++i) {
+      if (best) best[g + i] = h[(size_t)i * (1 + V)];
       if (elite)
-        std::memcpy(elite + (size_t)(g0 + g) * V, &h[(size_t)g * (1 + V) + 1], sizeof(float) * V);
+        std::memcpy(elite + (size_t)(g + i) * V, &h[(size_t)i * (1 + V) + 1], sizeof(float) * V);
     }
+    g += n;
   }
-  dga.download(ga, st);
+  dga.download(ga, generations & 1, st);
 }
 
 void Fitness::compute_wait(float* fitness, double* terms)

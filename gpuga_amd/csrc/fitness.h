@@ -25,6 +25,7 @@ struct FitnessOptions {
   int blocks_per_cu = 0;  // 0 = from the occupancy calculator
   int pack_lanes = 1;     // pack a last lane block of <= 32 individuals
   int queues = 0;         // work queues of the population launch; 0 = one per XCD
+  int ga_batch = 0;       // generations per staged batch of the device GA; 0 = up to 32
 };
 
 // Device-resident copy of one training set and its geometry tables.
@@ -68,11 +69,15 @@ struct PopPlan {
   PopCtl ctl{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  // device GA of this population size and one whole generation (population
-  // launch + GA step) as a graph
+  // device GA of this population size; runs of generations (population
+  // launch + GA step each) as graphs: ga_exec[k][j] is 2^j generations
+  // starting on population buffer k (a generation on buffer b writes the
+  // next into buffer 1 - b)
+  static constexpr int kGaGraphs = 6; // runs of 1, 2, 4, ..., 32 generations
   std::unique_ptr<DeviceGA> dga;
-  hipGraph_t gen_graph = nullptr;
-  hipGraphExec_t gen_exec = nullptr;
+  hipGraph_t ga_graph[2][kGaGraphs] = {};
+  hipGraphExec_t ga_exec[2][kGaGraphs] = {};
+  void drop_ga_graphs();
   ~PopPlan();
 };
 
@@ -133,7 +138,10 @@ private:
   void setup();
   PopPlan& plan(int P);
   void build_plan(PopPlan& pl, int P, int chunk_cost);
-  void launch_population(PopPlan& pl, hipStream_t st);
+  // the population launch on genes (default: the plan's buffer)
+  void launch_population(PopPlan& pl, hipStream_t st, const float* genes = nullptr);
+  // n generations of the device GA from population buffer k, captured
+  void capture_generations(PopPlan& pl, int k, int n, hipGraph_t* graph, hipGraphExec_t* exec);
   void launch_generation(PopPlan& pl, hipStream_t st); // graph replay or direct
   void capture(PopPlan& pl, hipStream_t st);           // the plan's graph, once
   double time_plan(int P, int chunk_cost, int reps);

@@ -214,15 +214,28 @@ int GA::get_a_parent()
   return 0;
 }
 
-void GA::crossover()
+void GA::draw_crossover(std::vector<int>& cross)
 {
-  // src/ga.cu:161-186
-  float* pop = population_.data();
+  // src/ga.cu:161-186: two parents (distinct), then the crossover point
   for (int m = 0; m < child_number_; m += 2) {
     const int p1 = get_a_parent();
     int p2 = get_a_parent();
     while (p2 == p1) p2 = get_a_parent();
     const int cp = rng_.uniform_int(1, V_ - 1);
+    cross.push_back(p1);
+    cross.push_back(p2);
+    cross.push_back(cp);
+  }
+}
+
+void GA::crossover()
+{
+  cross_.clear();
+  draw_crossover(cross_);
+  float* pop = population_.data();
+  for (int m = 0; m < child_number_; m += 2) {
+    const int* q = &cross_[(size_t)3 * (m / 2)];
+    const int p1 = q[0], p2 = q[1], cp = q[2];
     const int c1 = p_.parent_number + m, c2 = c1 + 1;
     for (int n = 0; n < cp; ++n) {
       pop[c1 * V_ + n] = pop[p1 * V_ + n];
@@ -235,18 +248,33 @@ void GA::crossover()
   }
 }
 
-void GA::mutation(int generation)
+int GA::mutation_count(int generation) const
 {
-  // src/ga.cu:188-198; the gene index is drawn before the value
+  // src/ga.cu:190-192 with the same float/double arithmetic
   const int m = p_.population_size * V_;
   const float ratio = (float)generation / (float)p_.maximum_generation;
   const float rate = (float)((double)p_.mutation_rate * (1.0 - (double)ratio));
-  const int count = (int)std::round((float)m * rate);
+  return std::max(0, (int)std::round((float)m * rate));
+}
+
+int GA::draw_mutation(int generation, std::vector<int>& idx, std::vector<float>& val)
+{
+  // src/ga.cu:193-197; the gene index is drawn before the value
+  const int m = p_.population_size * V_;
+  const int count = mutation_count(generation);
   for (int n = 0; n < count; ++n) {
-    const int idx = rng_.uniform_int(V_, m - 1);
-    const float val = rng_.canonical();
-    population_[idx] = val;
+    idx.push_back(rng_.uniform_int(V_, m - 1));
+    val.push_back(rng_.canonical());
   }
+  return count;
+}
+
+void GA::mutation(int generation)
+{
+  mut_idx_.clear();
+  mut_val_.clear();
+  const int count = draw_mutation(generation, mut_idx_, mut_val_);
+  for (int n = 0; n < count; ++n) population_[mut_idx_[n]] = mut_val_[n];
 }
 
 void GA::write_line(FILE* fid, int generation) const
@@ -291,76 +319,97 @@ void GA::run(const std::string& dir, const std::function<void(int, const float*,
 
 #include "ga_device.h"
 
+#include <algorithm>
+#include <climits>
+
 namespace gpuga {
 
-DeviceGA::DeviceGA(const GA& ga, float* genes, int history_cap, hipStream_t st)
+DeviceGA::DeviceGA(const GA& ga, float* genes0, int history_cap, hipStream_t st, int max_batch)
 {
   const GAParams& p = ga.params();
   const int P = p.population_size, V = ga.V();
   if (P > kDeviceGaMaxP)
     fail(strfmt("the device GA supports populations of at most %d", kDeviceGaMaxP));
+  ga_step_prepare(P, V);
   const size_t PV = (size_t)P * V;
-  if (!genes) {
+  if (!genes0) {
     genes_own_.resize(PV);
-    genes = genes_own_.get();
+    genes0 = genes_own_.get();
   }
+  genes0_ = genes0;
+  genes1_.resize(PV);
   sorted_.resize(P);
-  copy_.resize(PV);
-  cum_.upload(ga.cumulative(), st);
   hist_.resize((size_t)std::max(history_cap, 1) * (1 + V));
-  mt_.resize(624);
-  idx_.resize(1);
   gen_.resize(1);
-  last_.resize(PV);
-  GPUGA_HIP(hipMemsetAsync(last_.get(), 0xff, last_.bytes(), st)); // -1
-  dec_.resize(3 * (size_t)std::max((P - p.parent_number + 1) / 2, 1));
-  mut_idx_.resize(PV);
-  mut_val_.resize(PV);
-  mut_count_.resize(1);
+  hist_gen0_.resize(1);
+  counters_.resize(2); // steps | workgroups done
+  counters_.zero(st);
+  // a record holds at most the mutations of generation 0 (the count falls
+  // with the generation) and at most one per gene outside the best row
+  pairs_ = (P - p.parent_number) / 2;
+  max_mut_ = std::min(ga.mutation_count(0), (int)(PV - (size_t)V));
+  const size_t rec_max = 1 + 3 * (size_t)pairs_ + 2 * (size_t)max_mut_;
+  const size_t budget = (size_t)1 << 22; // words per batch
+  const size_t most = max_batch > 0 ? std::min(max_batch, kGaMaxBatch) : kGaMaxBatch;
+  const int batch = (int)std::max<size_t>(1, std::min<size_t>(most, budget / rec_max));
+  stage_words_ = (size_t)batch + (size_t)batch * rec_max;
+  if (stage_words_ > (size_t)INT_MAX) fail("device GA: draw records too large");
+  draws_.resize(stage_words_);
+  for (int k = 0; k < 2; ++k) {
+    host_[k].resize(stage_words_);
+    GPUGA_HIP(hipEventCreateWithFlags(&copied_[k], hipEventDisableTiming));
+  }
+  seen_.assign(PV, 0);
+  int rows = 0;
+  size_t lds = 0;
+  ga_step_shape(P, V, &rows, &lds);
   d_.P = P;
   d_.V = V;
   d_.parents = p.parent_number;
   d_.children = P - p.parent_number;
-  d_.max_gen = p.maximum_generation;
-  d_.mutation_rate = p.mutation_rate;
-  d_.genes = genes;
+  d_.rows = rows;
+  d_.genes_in = genes0_;
+  d_.genes_out = genes1_.get();
   d_.fitness = nullptr;
   d_.sorted = sorted_.get();
-  d_.genes_copy = copy_.get();
-  d_.cumulative = cum_.get();
-  d_.mt = mt_.get();
-  d_.mt_idx = idx_.get();
+  d_.draws = draws_.get();
+  d_.batch = batch;
+  d_.step = counters_.get();
+  d_.done = counters_.get() + 1;
   d_.generation = gen_.get();
   d_.history = hist_.get();
   d_.history_cap = std::max(history_cap, 0);
-  hist_gen0_.resize(1);
   d_.history_gen0 = hist_gen0_.get();
-  d_.last = last_.get();
-  d_.dec = dec_.get();
-  d_.mut_idx = mut_idx_.get();
-  d_.mut_val = mut_val_.get();
-  d_.mut_count = mut_count_.get();
+  max_gen_ = p.maximum_generation;
+  rate_ = p.mutation_rate;
+  GPUGA_HIP(hipStreamSynchronize(st));
+}
+
+DeviceGA::~DeviceGA()
+{
+  for (int k = 0; k < 2; ++k)
+    if (copied_[k]) (void)hipEventDestroy(copied_[k]);
 }
 
 bool DeviceGA::matches(const GA& ga) const
 {
   const GAParams& p = ga.params();
   return d_.P == p.population_size && d_.V == ga.V() && d_.parents == p.parent_number &&
-         d_.max_gen == p.maximum_generation && d_.mutation_rate == p.mutation_rate;
+         max_gen_ == p.maximum_generation && rate_ == p.mutation_rate;
 }
 
 void DeviceGA::upload(const GA& ga, int generation, hipStream_t st)
 {
+  if (generation < 0) fail("generation numbers start at 0");
   const size_t PV = (size_t)d_.P * d_.V;
-  GPUGA_HIP(hipMemcpyAsync(d_.genes, ga.population().data(), sizeof(float) * PV,
+  GPUGA_HIP(hipMemcpyAsync(genes0_, ga.population().data(), sizeof(float) * PV,
                            hipMemcpyHostToDevice, st));
-  GPUGA_HIP(hipMemcpyAsync(d_.mt, ga.rng().state(), sizeof(uint32_t) * 624,
-                           hipMemcpyHostToDevice, st));
-  const int idx = ga.rng().index();
-  GPUGA_HIP(hipMemcpyAsync(d_.mt_idx, &idx, sizeof(int), hipMemcpyHostToDevice, st));
+  const unsigned zero[2] = {0u, 0u};
+  GPUGA_HIP(hipMemcpyAsync(d_.step, zero, sizeof zero, hipMemcpyHostToDevice, st));
   GPUGA_HIP(hipMemcpyAsync(d_.generation, &generation, sizeof(int), hipMemcpyHostToDevice, st));
   GPUGA_HIP(hipMemcpyAsync(d_.history_gen0, &generation, sizeof(int), hipMemcpyHostToDevice, st));
   GPUGA_HIP(hipStreamSynchronize(st)); // the host values are stack/host memory
+  staged_ = 0;
 }
 
 void DeviceGA::restart_history(int generation, hipStream_t st)
@@ -369,26 +418,71 @@ void DeviceGA::restart_history(int generation, hipStream_t st)
   GPUGA_HIP(hipStreamSynchronize(st));
 }
 
-void DeviceGA::download(GA& ga, hipStream_t st) const
+void DeviceGA::stage(GA& ga, int generation, int n, hipStream_t st)
+{
+  const int B = d_.batch;
+  if (n < 1 || n > B) fail(strfmt("device GA: %d records staged, batches hold 1-%d", n, B));
+  if (staged_ % B != 0) fail("device GA: records after a partial batch");
+  if (generation < 0) fail("generation numbers start at 0");
+  const int k = next_;
+  next_ ^= 1;
+  // the staging area's previous copy has run (it was queued two batches ago)
+  if (pending_[k]) GPUGA_HIP(hipEventSynchronize(copied_[k]));
+  int* h = host_[k].get();
+  size_t w = (size_t)B;
+  for (int i = 0; i < n; ++i) {
+    h[i] = (int)w;
+    cross_.clear();
+    idx_.clear();
+    val_.clear();
+    ga.draw_crossover(cross_);
+    const int count = ga.draw_mutation(generation + i, idx_, val_);
+    int* r = h + w;
+    std::memcpy(r + 1, cross_.data(), sizeof(int) * 3 * (size_t)pairs_);
+    // the last mutation of a gene is the one that stays: walk backwards and
+    // keep each gene's first sighting
+    if (++seen_tag_ == INT_MAX) {
+      std::fill(seen_.begin(), seen_.end(), 0);
+      seen_tag_ = 1;
+    }
+    int* ri = r + 1 + 3 * pairs_;
+    keep_.clear();
+    for (int j = count - 1; j >= 0; --j) {
+      const int g = idx_[j];
+      if (seen_[g] == seen_tag_) continue;
+      seen_[g] = seen_tag_;
+      ri[keep_.size()] = g;
+      keep_.push_back(val_[j]);
+    }
+    const int m = (int)keep_.size();
+    if (m > max_mut_) fail("device GA: more mutations than the staging area holds");
+    std::memcpy(ri + m, keep_.data(), sizeof(float) * (size_t)m);
+    r[0] = m;
+    w += 1 + 3 * (size_t)pairs_ + 2 * (size_t)m;
+  }
+  GPUGA_HIP(hipMemcpyAsync(draws_.get(), h, sizeof(int) * w, hipMemcpyHostToDevice, st));
+  GPUGA_HIP(hipEventRecord(copied_[k], st));
+  pending_[k] = true;
+  staged_ += n;
+}
+
+void DeviceGA::step(int k, const float* fitness, hipStream_t st) const
+{
+  GaDev d = d_;
+  d.genes_in = genes(k);
+  d.genes_out = genes(1 - k);
+  d.fitness = fitness;
+  launch_ga_step(d, st);
+}
+
+void DeviceGA::download(GA& ga, int k, hipStream_t st) const
 {
   const size_t PV = (size_t)d_.P * d_.V;
-  uint32_t s[624];
-  int idx = 0;
-  GPUGA_HIP(hipMemcpyAsync(ga.population().data(), d_.genes, sizeof(float) * PV,
+  GPUGA_HIP(hipMemcpyAsync(ga.population().data(), genes(k), sizeof(float) * PV,
                            hipMemcpyDeviceToHost, st));
   GPUGA_HIP(hipMemcpyAsync(ga.fitness().data(), d_.sorted, sizeof(float) * d_.P,
                            hipMemcpyDeviceToHost, st));
-  GPUGA_HIP(hipMemcpyAsync(s, d_.mt, sizeof s, hipMemcpyDeviceToHost, st));
-  GPUGA_HIP(hipMemcpyAsync(&idx, d_.mt_idx, sizeof idx, hipMemcpyDeviceToHost, st));
   GPUGA_HIP(hipStreamSynchronize(st));
-  ga.rng().set_state(s, idx);
-}
-
-void DeviceGA::step(const float* fitness, hipStream_t st)
-{
-  GaDev d = d_;
-  d.fitness = fitness;
-  launch_ga_generation(d, st);
 }
 
 void DeviceGA::history(float* out, int n, hipStream_t st) const

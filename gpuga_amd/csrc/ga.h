@@ -57,6 +57,19 @@ public:
   void sort_population();
   void crossover();
   void mutation(int generation);
+  // The random decisions of crossover and mutation, drawn from the stream in
+  // the order crossover() and mutation() draw them (they use these).  They
+  // depend on the stream and the generation number, not on the fitness, so a
+  // run can draw them ahead of the evaluations.
+  // crossover (src/ga.cu:161-186): per child pair, the parent ranks p1, p2
+  // and the crossover point, appended to cross
+  void draw_crossover(std::vector<int>& cross);
+  // mutation (src/ga.cu:188-198): gene indices and values in draw order,
+  // appended to idx and val; returns their number
+  int draw_mutation(int generation, std::vector<int>& idx, std::vector<float>& val);
+  // number of mutations of a generation
+  int mutation_count(int generation) const;
+  int children() const { return child_number_; }
   // the reference's ga.out line (src/ga.cu:141-147)
   void write_line(FILE* fid, int generation) const;
   // full run as GA::GA + GA::compute (src/ga.cu:27-92); fitness callback
@@ -81,6 +94,8 @@ private:
   std::vector<float> pmin_, pmax_;
   std::vector<float> population_, population_copy_, fitness_, cumulative_;
   std::vector<int> index_;
+  std::vector<int> cross_, mut_idx_;
+  std::vector<float> mut_val_;
   MT19937 rng_;
 };
 

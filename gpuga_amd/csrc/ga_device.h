@@ -1,5 +1,5 @@
 // The GA's generation step on the device (ga_device.hip) and the host object
-// that holds its state next to a population buffer.
+// that holds its state next to the population buffers.
 #pragma once
 
 #include <cstdint>
@@ -12,61 +12,98 @@ namespace gpuga {
 class GA;
 
 // Device view of the GA state for one generation step (all device pointers).
+//
+// Draw records: the random decisions of one generation, drawn on the host
+// (GA::draw_crossover / GA::draw_mutation) and staged in a batch buffer of
+// `batch` records: words [0, batch) are record offsets (in words, from the
+// buffer start), a record is
+//   [0]                      n = mutations after removing overwritten ones
+//   [1, 1 + 3 * pairs)       crossover: parent rank p1, p2, crossover point
+//   [.., + n)                gene index of each mutation
+//   [.., + n)                its value (float bits)
+// Step s after the upload reads record s mod batch.
 struct GaDev {
-  int P, V, parents, children, max_gen;
-  float mutation_rate;
-  float* genes;            // [P][V] population in [0, 1], updated in place
-  const float* fitness;    // [P] fitness of the population
+  int P, V, parents, children;
+  int rows;                // population rows per workgroup
+  const float* genes_in;   // [P][V] population of this generation, in [0, 1]
+  float* genes_out;        // [P][V] the next generation
+  const float* fitness;    // [P] fitness of genes_in
   float* sorted;           // [P] fitness in sorted order (out)
-  float* genes_copy;       // [P][V] scratch
-  const float* cumulative; // [parents] rank-selection probabilities
-  uint32_t* mt;            // [624] std::mt19937 state words
-  int* mt_idx;             // [1] position in the state block
+  const int* draws;        // batch buffer of draw records
+  int batch;               // records per batch
+  unsigned* step;          // [1] steps since the upload
+  unsigned* done;          // [1] workgroups finished, 0 between steps
   int* generation;         // [1] generation number, advanced by each step
   float* history;          // [history_cap][1 + V] best fitness and elite row
   int history_cap;         // rows; generation g goes to row g - *history_gen0
   int* history_gen0;       // [1]
-  int* last;               // [P][V] scratch, -1 between steps
-  int* dec;                // [children / 2][3] crossover parents and point
-  int* mut_idx;            // [P * V] mutation list: gene index ...
-  float* mut_val;          // [P * V] ... and value, in draw order
-  int* mut_count;          // [1]
 };
 
-// Largest population the one-workgroup step supports (LDS budget).
+// Largest population the step supports (LDS budget of its sort).
 constexpr int kDeviceGaMaxP = 4096;
+// Most draw records in one staged batch.
+constexpr int kGaMaxBatch = 32;
 
-size_t ga_device_lds_bytes(int P, int parents, int children);
-void launch_ga_generation(const GaDev& d, hipStream_t st);
+// rows per workgroup and LDS bytes of the step for shape (P, V)
+void ga_step_shape(int P, int V, int* rows, size_t* lds);
+// checks the shape fits a workgroup and raises the LDS limit when needed
+// (once per shape, outside graph capture)
+void ga_step_prepare(int P, int V);
+void launch_ga_step(const GaDev& d, hipStream_t st);
 
-// Device copy of a host GA (population, random state, selection table) for a
-// run of generations without host round trips.  genes may be an external
-// device buffer (the population kernel's input) or owned here.
+// Device state of a host GA for runs of generations without host round
+// trips: two population buffers (a step reads one and writes the other),
+// the sorted fitness, the ga.out history, the staged draw records.  Buffer 0
+// may be an external device buffer (the population kernel's input).
 class DeviceGA {
 public:
-  DeviceGA(const GA& ga, float* genes, int history_cap, hipStream_t st);
+  // max_batch: most generations per staged batch (0: up to kGaMaxBatch)
+  DeviceGA(const GA& ga, float* genes0, int history_cap, hipStream_t st, int max_batch = 0);
+  ~DeviceGA();
+  DeviceGA(const DeviceGA&) = delete;
+  DeviceGA& operator=(const DeviceGA&) = delete;
   // whether this state was built for a GA of ga's shape and settings
   bool matches(const GA& ga) const;
-  // host GA -> device (population, random state, generation number); the
+  // host population -> buffer 0, generation number, step count 0; the
   // history restarts at that generation
   void upload(const GA& ga, int generation, hipStream_t st);
   // the history restarts at `generation` (for runs longer than its capacity)
   void restart_history(int generation, hipStream_t st);
   int history_cap() const { return d_.history_cap; }
-  // device -> host GA (population, random state, sorted fitness)
-  void download(GA& ga, hipStream_t st) const;
-  // one generation step on fitness (device pointer)
-  void step(const float* fitness, hipStream_t st);
-  // history rows [n][1 + V] of the generations since upload
+  int batch() const { return d_.batch; }
+  // Draw the records of `n` <= batch() generations from generation
+  // `generation` on the host GA (advancing its random stream as the host
+  // steps would) and queue their copy to the device on st.  The steps that
+  // read them must be queued after this call and before the next stage().
+  void stage(GA& ga, int generation, int n, hipStream_t st);
+  // one step reading population buffer k and writing buffer 1 - k, on
+  // fitness (device pointer); graph-capturable
+  void step(int k, const float* fitness, hipStream_t st) const;
+  // population buffer k and sorted fitness -> host GA (its random stream is
+  // already where the staged steps leave it)
+  void download(GA& ga, int k, hipStream_t st) const;
+  // history rows [n][1 + V] of the generations since the last restart
   void history(float* out, int n, hipStream_t st) const;
-  const GaDev& view() const { return d_; }
-  float* genes() const { return d_.genes; }
+  float* genes(int k) const { return k ? genes1_.get() : genes0_; }
 
 private:
   GaDev d_{};
-  DeviceBuffer<float> genes_own_, sorted_, copy_, cum_, hist_, fit_, mut_val_;
-  DeviceBuffer<uint32_t> mt_;
-  DeviceBuffer<int> idx_, gen_, last_, dec_, mut_idx_, mut_count_, hist_gen0_;
+  int pairs_ = 0, max_mut_ = 0, max_gen_ = 0;
+  float rate_ = 0.0f;
+  size_t stage_words_ = 0;
+  long staged_ = 0; // records staged since the upload
+  float* genes0_ = nullptr;
+  DeviceBuffer<float> genes_own_, genes1_, sorted_, hist_;
+  DeviceBuffer<int> draws_, gen_, hist_gen0_;
+  DeviceBuffer<unsigned> counters_;
+  // two host staging areas, each reusable once its copy has run
+  PinnedBuffer<int> host_[2];
+  hipEvent_t copied_[2] = {nullptr, nullptr};
+  bool pending_[2] = {false, false};
+  int next_ = 0;
+  std::vector<int> cross_, idx_, seen_;
+  std::vector<float> val_, keep_;
+  int seen_tag_ = 0;
 };
 
 } // namespace gpuga

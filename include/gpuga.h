@@ -33,6 +33,8 @@ typedef struct gpuga_options {
   int pack_lanes;    /* 1: a last lane block of <= 32 individuals runs several
                         atoms per wave (three-body potentials); 0: one atom */
   int queues;        /* work queues of the population launch; 0 = one per XCD */
+  int ga_batch;      /* generations per staged batch of gpuga_fitness_run_ga's
+                        random decisions; 0 = up to 32 */
 } gpuga_options;
 
 /* Thread-local text of the last failure (src/error.cu:21-25 prints it). */

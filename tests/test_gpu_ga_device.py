@@ -2,11 +2,13 @@
 bit-exact to the oracle and libstdc++ <random>): the same populations, sorted
 fitness and random state, generation after generation.
 
-Shapes cover the selection table, crossover and mutation paths: with
-P = 256, V = 1153 (the NN2B n = 32 layout) a gene-index draw is rejected
-with probability 290191 / 2^32 = 6.8e-5, about 4 times in each early
-generation's 59034 mutations, so the redraw path between the parallel
-mutation batches runs many times."""
+The device step takes the random decisions the host draws for it (staged
+draw records) and does the sort, crossover and mutation.  Shapes cover the
+rank sort (P <= 512) and the bitonic sort (P > 512) with many equal
+fitness values, the one-thread insertion sort when a NaN is present, a
+population split over many workgroups (P = 256, V = 1153, the NN2B n = 32
+layout: 7 rows per workgroup, 59034 mutations per early generation, many of
+them on the same gene), and the largest population (4096)."""
 import numpy as np
 import pytest
 
@@ -34,6 +36,9 @@ def fitness_seq(P, gens, seed, ties=False, nan_at=None):
     (64, 32, 141, 20, False, 3),
     (130, 64, 15, 20, True, 5),
     (256, 128, 1153, 6, False, None),
+    (1000, 500, 9, 8, True, None),
+    (600, 300, 15, 6, False, 2),
+    (4096, 2048, 12, 3, True, None),
 ])
 def test_device_step_equals_host_step(P, parents, V, gens, ties, nan_at):
     import gpuga_amd as G
@@ -74,4 +79,25 @@ def test_run_ga_equals_host_loop(golden, name, P, parents):
     s = host.step(gens, f)
     b2, e2 = fit.run_ga(dev, gens, 1)
     assert b2[0] == s[0]
+    assert np.array_equal(host.population, dev.population)
+
+
+@pytest.mark.parametrize("batch,gens", [(0, 70), (5, 23)])
+def test_run_ga_across_staging_batches(golden, batch, gens):
+    """A run longer than one batch of staged draw records (32 generations
+    by default; 5 forces batches that start on either population buffer)
+    alternates population buffers, replays whole-batch graphs and
+    single-generation graphs for the rest, and restages; it equals the host
+    loop."""
+    import gpuga_amd as G
+    print_case("run_ga batches", "hfo2", 40, batch, gens)
+    fit = G.Fitness(golden("hfo2"), ga_batch=batch)
+    host = G.GA(40, 20, fit.V, maximum_generation=gens, mutation_rate=0.2, seed=99)
+    dev = G.GA(40, 20, fit.V, maximum_generation=gens, mutation_rate=0.2, seed=99)
+    best_h = []
+    for g in range(gens):
+        s = host.step(g, fit.compute(host.population))
+        best_h.append(s[0])
+    best_d, _ = fit.run_ga(dev, 0, gens)
+    assert np.array_equal(np.array(best_h, np.float32), best_d)
     assert np.array_equal(host.population, dev.population)
