@@ -53,17 +53,16 @@ __host__ __device__ inline TLane tersoff_lane(const TersoffParams& p)
 // fc and dfc/dr (src/minimal_tersoff.cu:77-90)
 __device__ __forceinline__ void tersoff_fc(const TLane& q, float d, float& fc, float& fcp)
 {
-  fc = 1.0f;
+  // 1 below r1, 0 from r2 on (a select), the taper between them under one
+  // branch that the wave skips when no lane needs it
+  const bool below = d < q.r1;
+  fc = below ? 1.0f : 0.0f;
   fcp = 0.0f;
-  if (d >= q.r1) {
-    if (d < q.r2) {
-      float sn, cs;
-      sincospi01((d - q.r1) * q.inv_span, sn, cs);
-      fc = 0.5f * cs + 0.5f;
-      fcp = -q.pif_half * sn;
-    } else {
-      fc = 0.0f;
-    }
+  if (!below && d < q.r2) {
+    float sn, cs;
+    sincospi01((d - q.r1) * q.inv_span, sn, cs);
+    fc = 0.5f * cs + 0.5f;
+    fcp = -q.pif_half * sn;
   }
 }
 
@@ -242,14 +241,35 @@ struct TersoffPop {
   __device__ static void atom(const DevSet& s, const TLane& q, int i, float* fblk, int lane,
                               Acc& acc)
   {
+    if constexpr (U) i = __builtin_amdgcn_readfirstlane(i);
     const float4* R = s.rec + (long)i * RS;
-    // Uniform records: every load is unconditional (records are zero-padded
-    // to the bound), so the loads of a row issue together and are waited for
-    // once, instead of one load and wait per guarded use.  Per-lane records
-    // (packed items) are loaded at each use instead, which keeps them out of
-    // VGPRs.
-    const int4 hd = ldx<U>(reinterpret_cast<const int4*>(R), 0);
-    constexpr bool PRE = U && NMAX <= 8; // preload rows (SGPR budget allows)
+    // header words as int loads (an int4 load through the constant address
+    // space is issued as a vector load, waited for at every atom)
+    const int* H = reinterpret_cast<const int*>(R);
+    const int nn = ldx<U>(H, 0), unl = TWO ? ldx<U>(H, 1) : 0;
+    // an atom with the full neighbour bound (every atom of diamond Si) runs
+    // a body without neighbour-count guards: no selects, no guarded loads
+    if constexpr (U) {
+      if (nn == NMAX) {
+        atom_body<FORCE, U, true>(s, q, i, R, nn, unl, fblk, lane, acc);
+        return;
+      }
+    }
+    atom_body<FORCE, U, false>(s, q, i, R, nn, unl, fblk, lane, acc);
+  }
+
+  template <bool FORCE, bool U, bool FULL>
+  __device__ static void atom_body(const DevSet& s, const TLane& q, int i, const float4* R,
+                                   int nn, int unl, float* fblk, int lane, Acc& acc)
+  {
+    // Every load of a row is unconditional (records are zero-padded to the
+    // bound), so the loads issue together and are waited for once, instead
+    // of one load and wait per guarded use.  Uniform records (U) land in
+    // SGPRs; per-lane records (packed items) in VGPRs, which the packed path
+    // can afford: loaded at each use, inside the per-lane branches, every
+    // load waited for on its own made a packed atom step cost five uniform
+    // ones.
+    constexpr bool PRE = NMAX <= 8; // preload rows (register budget allows)
     float d[NMAX], dinv[NMAX];
     if constexpr (PRE) {
 #pragma unroll
@@ -262,11 +282,10 @@ struct TersoffPop {
     auto dist_inv = [&](int u) {
       return PRE ? dinv[u] : ldx<U>(reinterpret_cast<const float*>(R + INV), u);
     };
-    const int nn = hd.x, unl = hd.y;
     float fc[NMAX], fcp[NMAX], fcb[NMAX], A[NMAX], B[NMAX], pf[NMAX];
 #pragma unroll
     for (int u = 0; u < NMAX; ++u) {
-      if (u < nn) {
+      if (FULL || u < nn) {
         tersoff_fc(q, dist(u), fc[u], fcp[u]);
         fcb[u] = fc[u] * q.beta;
       }
@@ -274,7 +293,7 @@ struct TersoffPop {
     float e = 0.0f;
 #pragma unroll
     for (int u = 0; u < NMAX; ++u) {
-      if (u < nn) {
+      if (FULL || u < nn) {
         float cs[NMAX];
         if constexpr (PRE) {
 #pragma unroll
@@ -284,7 +303,7 @@ struct TersoffPop {
         float zeta = 0.0f;
 #pragma unroll
         for (int v = 0; v < NMAX; ++v) {
-          if (v < nn && v != u) {
+          if ((FULL || v < nn) && v != u) {
             const float x = (PRE ? cs[v] : ldx<U>(R, 1 + NMAX + u * NMAX + v).x) - q.h;
             zeta = fmaf(fcb[v], x * x, zeta);
           }
@@ -309,7 +328,7 @@ struct TersoffPop {
     float wxx = 0, wyy = 0, wzz = 0, wxy = 0, wyz = 0, wzx = 0;
 #pragma unroll
     for (int u = 0; u < NMAX; ++u) {
-      if (u < nn) {
+      if (FULL || u < nn) {
         // three-body part of f_ij with beta factored out of both sums:
         //   c    = sum_k (h - cos)(A_j + A_k) fc_k dcos/dr   (times 2 beta)
         //   dsum = sum_k B_k (h - cos)^2                     (times beta)
@@ -323,7 +342,7 @@ struct TersoffPop {
         float cx = 0.0f, cy = 0.0f, cz = 0.0f, dsum = 0.0f;
 #pragma unroll
         for (int v = 0; v < NMAX; ++v) {
-          if (v < nn && v != u) {
+          if ((FULL || v < nn) && v != u) {
             const float4 t = PRE ? row[v] : ldx<U>(R, 1 + NMAX + u * NMAX + v);
             const float x = t.x - q.h;
             const float w = fmaf(A[u], fc[v], B[v]) * x;
@@ -426,13 +445,14 @@ struct TersoffPop {
 #pragma unroll
       for (int h = 0; h < STEP; ++h) {
         const int ii = min(i + h, a1 - 1);
-        const int4 hd = ldu(reinterpret_cast<const int4*>(s.rec + (long)ii * RS), 0);
+        const int* H = reinterpret_cast<const int*>(s.rec + (long)ii * RS);
+        const int nn = ldu(H, 0), b0 = ldu(H, 2);
         const float* own = fblk + ((size_t)s.bond_force + ii) * 3 * WAVE;
         float fx = cload(own), fy = cload(own + WAVE), fz = cload(own + 2 * WAVE);
 #pragma unroll
         for (int u = 0; u < NMAX; ++u) {
-          const bool on = u < hd.x;
-          const float* r = fblk + (size_t)(on ? hd.z + u : hd.z) * 3 * WAVE;
+          const bool on = u < nn;
+          const float* r = fblk + (size_t)(on ? b0 + u : b0) * 3 * WAVE;
           const float rx = cload(r), ry = cload(r + WAVE), rz = cload(r + 2 * WAVE);
           fx -= on ? rx : 0.0f;
           fy -= on ? ry : 0.0f;

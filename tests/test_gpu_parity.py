@@ -116,12 +116,12 @@ def test_population_sizes(G, golden, oracle_case, P):
 
 
 @pytest.mark.parametrize("name,P", [("si_diamond", 72), ("si_diamond", 65), ("si_diamond", 8),
-                                    ("gaas", 90), ("si_type2", 130)])
+                                    ("gaas", 76), ("si_type2", 130)])
 def test_packed_lane_block_matches_unpacked(G, golden, name, P):
-    """A last lane block of <= 32 individuals runs several atoms per wave (its
-    own queue and kernel); its fitness and terms must equal the one-atom-per-
-    wave form (same FP32 formulas, FP64 sums regrouped), for the individuals
-    of that block and the full blocks alike."""
+    """A last lane block of <= 16 individuals runs several atoms per wave;
+    its fitness and terms must equal the one-atom-per-wave form (same FP32
+    formulas, FP64 sums regrouped), for the individuals of that block and the
+    full blocks alike."""
     print_case("packed lanes", name, P)
     packed = product(G, golden, name)
     plain = product(G, golden, name, pack_lanes=0)
