@@ -23,7 +23,7 @@ namespace gpuga {
 // 0.5 pi/(R2-R1) for the taper.
 struct TLane {
   float cR[2], kR[2], kR2[2], cA[2], kA[2], kA2[2], r0[2];
-  float en, mhon, beta, beta2, h, r1, r2, inv_span, pif_half;
+  float en, mhon, en1, mhon1, beta, beta2, h, r1, r2, inv_span, pif_half;
 };
 
 __host__ __device__ inline TLane tersoff_lane(const TersoffParams& p)
@@ -40,6 +40,8 @@ __host__ __device__ inline TLane tersoff_lane(const TersoffParams& p)
   }
   q.en = p.en;
   q.mhon = p.minus_half_over_n;
+  q.en1 = p.en - 1.0f;
+  q.mhon1 = p.minus_half_over_n - 1.0f;
   q.beta = p.beta;
   q.beta2 = 2.0f * p.beta;
   q.h = p.h;
@@ -66,19 +68,19 @@ __device__ __forceinline__ void tersoff_fc(const TLane& q, float d, float& fc, f
   }
 }
 
-// b = (1 + zeta^n)^(-1/2n) and db/dzeta; b = 1, b' = 0 below zeta = 1e-16
-// (src/minimal_tersoff.cu:174-185)
+// b = (1 + zeta^n)^(-1/2n) and db/dzeta = -1/2 (1 + zeta^n)^(-1/2n - 1)
+// zeta^(n - 1), the latter as one exp2 of the summed logarithms (no
+// reciprocal); b = 1, b' = 0 below zeta = 1e-16 (src/minimal_tersoff.cu:174-185)
 __device__ __forceinline__ void tersoff_bond_order(const TLane& q, float zeta, float& b, float& bp)
 {
-  if (zeta < 1.0e-16f) {
-    b = 1.0f;
-    bp = 0.0f;
-  } else {
-    const float bzn = fexp2(q.en * flog2(zeta));
-    const float one = 1.0f + bzn;
-    b = fexp2(q.mhon * flog2(one));
-    bp = -0.5f * b * bzn * frcp(one * zeta);
-  }
+  const float lz = flog2(zeta);
+  const float bzn = fexp2(q.en * lz);
+  const float l1 = flog2(1.0f + bzn);
+  // the cut-off as selects of the exponents (exp2(0) = 1, exp2(-inf) = 0),
+  // so that no branch forms around the exp2 instructions
+  const bool tiny = zeta < 1.0e-16f;
+  b = fexp2(tiny ? 0.0f : q.mhon * l1);
+  bp = -fexp2(tiny ? -__builtin_inff() : fmaf(q.mhon1, l1, fmaf(q.en1, lz, -1.0f)));
 }
 
 // fR, fR', fA, fA' (src/minimal_tersoff.cu:62-75) of parameter set k
@@ -290,12 +292,29 @@ struct TersoffPop {
         fcb[u] = fc[u] * q.beta;
       }
     }
+    // With four bonds, x = cos(theta_jk) - h and x^2 once per unordered pair:
+    // the record holds the same cosine, bit for bit, at (j, k) and (k, j)
+    // (the set-up computes it with commuting products in one order).
+    constexpr bool SYM = NMAX <= 4;
+    float xs[NMAX][NMAX], x2[NMAX][NMAX];
+    if constexpr (SYM) {
+#pragma unroll
+      for (int u = 0; u < NMAX; ++u)
+#pragma unroll
+        for (int v = u + 1; v < NMAX; ++v)
+          if (FULL || v < nn) {
+            xs[u][v] = ldx<U>(R, 1 + NMAX + u * NMAX + v).x - q.h;
+            x2[u][v] = xs[u][v] * xs[u][v];
+          }
+    }
+    auto xsym = [&](int u, int v) { return u < v ? xs[u][v] : xs[v][u]; };
+    auto x2sym = [&](int u, int v) { return u < v ? x2[u][v] : x2[v][u]; };
     float e = 0.0f;
 #pragma unroll
     for (int u = 0; u < NMAX; ++u) {
       if (FULL || u < nn) {
         float cs[NMAX];
-        if constexpr (PRE) {
+        if constexpr (PRE && !SYM) {
 #pragma unroll
           for (int v = 0; v < NMAX; ++v)
             if (v != u) cs[v] = ldx<U>(R, 1 + NMAX + u * NMAX + v).x;
@@ -304,8 +323,12 @@ struct TersoffPop {
 #pragma unroll
         for (int v = 0; v < NMAX; ++v) {
           if ((FULL || v < nn) && v != u) {
-            const float x = (PRE ? cs[v] : ldx<U>(R, 1 + NMAX + u * NMAX + v).x) - q.h;
-            zeta = fmaf(fcb[v], x * x, zeta);
+            if constexpr (SYM) {
+              zeta = fmaf(fcb[v], x2sym(u, v), zeta);
+            } else {
+              const float x = (PRE ? cs[v] : ldx<U>(R, 1 + NMAX + u * NMAX + v).x) - q.h;
+              zeta = fmaf(fcb[v], x * x, zeta);
+            }
           }
         }
         float b, bp;
@@ -344,12 +367,13 @@ struct TersoffPop {
         for (int v = 0; v < NMAX; ++v) {
           if ((FULL || v < nn) && v != u) {
             const float4 t = PRE ? row[v] : ldx<U>(R, 1 + NMAX + u * NMAX + v);
-            const float x = t.x - q.h;
+            const float x = SYM ? xsym(u, v) : t.x - q.h;
             const float w = fmaf(A[u], fc[v], B[v]) * x;
             cx = fmaf(w, t.y, cx);
             cy = fmaf(w, t.z, cy);
             cz = fmaf(w, t.w, cz);
-            dsum = fmaf(B[v] * x, x, dsum);
+            if constexpr (SYM) dsum = fmaf(B[v], x2sym(u, v), dsum);
+            else dsum = fmaf(B[v] * x, x, dsum);
           }
         }
         const float4 r = ldx<U>(R, 1 + u);
