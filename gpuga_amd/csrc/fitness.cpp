@@ -181,7 +181,7 @@ void Fitness::setup()
   if (spec_.type == POT_RI)
     for (int t : ts_.type)
       if (t != 0 && t != 1) fail("RI potential needs atom types 0 and 1");
-  nl_ = build_neighbors(ts_, spec_.cutoff);
+  nl_ = build_neighbors_device(ts_, spec_.cutoff, stream_);
   for (int i = 0; i < ts_.N; ++i)
     for (int s = 0; s < nl_.NN[i]; ++s)
       if (nl_.NL[nl_.rev[nl_.start[i] + s]] != i)
@@ -501,6 +501,15 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   }
   std::stable_sort(chunks.begin(), chunks.end(),
                    [](const Chunk& a, const Chunk& b) { return a.cost > b.cost; });
+  // Three-body force configurations first: their force-assembly items then
+  // run while the energy configurations' chunks are still being served, and
+  // the end of the launch is the smallest energy chunks, which wait for
+  // nothing (largest first within each part).  Ordered by size alone, small
+  // force configurations came last and their assembly items formed a tail:
+  // 20% of the launch with fewer and fewer waves busy.
+  if (tersoff)
+    std::stable_partition(chunks.begin(), chunks.end(),
+                          [&](const Chunk& a) { return a.c < ts_.Nc_force; });
   // A last lane block of at most 16 individuals runs packed: G >= 4 groups
   // of W lanes, each group on every G-th atom of the chunk.  (A packed atom
   // step costs several uniform ones, since its geometry is per lane: with
@@ -983,18 +992,33 @@ int Fitness::heuristic_chunk_cost(int P) const
 size_t Fitness::estimate_bytes(const PotentialSpec& s, const TrainSet& ts, int P,
                                const FitnessOptions& opt)
 {
-  const NeighborList nl = build_neighbors(ts, s.cutoff);
-  const size_t N = ts.N, Nc = ts.Nc, B = std::max(nl.total, 1);
-  const bool tersoff = s.type == POT_TERSOFF1 || s.type == POT_TERSOFF2;
+  // neighbour counts on the device (a counting pass of the list build)
+  hipStream_t st = nullptr;
+  GPUGA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  std::vector<int> nn;
+  try {
+    nn = count_neighbors_device(ts, s.cutoff, st);
+  } catch (...) {
+    (void)hipStreamDestroy(st);
+    throw;
+  }
+  GPUGA_HIP(hipStreamDestroy(st));
+  long long bonds = 0;
   size_t tri = 0;
-  for (int i = 0; i < ts.N; ++i) tri += (size_t)nl.NN[i] * nl.NN[i];
+  for (int i = 0; i < ts.N; ++i) {
+    bonds += nn[i];
+    tri += (size_t)nn[i] * nn[i];
+  }
+  const size_t N = ts.N, Nc = ts.Nc, B = std::max<long long>(bonds, 1);
+  const bool tersoff = s.type == POT_TERSOFF1 || s.type == POT_TERSOFF2;
   size_t b = N * sizeof(int) * 5 + Nc * sizeof(int) * 2 + B * (3 * sizeof(int) + 16);
   b += (tersoff ? tri * 16 + B * 4 * 5 : 0) + (s.type == POT_RI ? B * 48 + N * 16 : 0);
   b += s.type == POT_NN2B ? B * 48 + N * 16 : 0;
   b += Nc * 7 * sizeof(float) + N * 3 * sizeof(float);  // references
   b += N * 10 * sizeof(float) + Nc * 3 * sizeof(double); // per-atom outputs
   const size_t K = (P + WAVE - 1) / WAVE;
-  const size_t B_force = ts.N_force > 0 ? (size_t)nl.start[ts.N_force - 1] + nl.NN[ts.N_force - 1] : 0;
+  size_t B_force = 0;
+  for (int i = 0; i < ts.N_force; ++i) B_force += nn[i];
   b += K * 3 * Nc * WAVE * sizeof(double) + 2 * Nc * K * 8 * WAVE * sizeof(double);
   if (tersoff) b += K * (B_force + ts.N_force) * 3 * WAVE * sizeof(float);
   if (s.type == POT_NN2B) b += (size_t)3 * opt.nn2b_grid * K * WAVE * sizeof(float);

@@ -1,0 +1,176 @@
+// Neighbour list on the device (the reference builds it on the GPU as well,
+// src/neighbor.cu:30-66), in the CSR form of build_neighbors (trainset.cpp)
+// and bit-identical to it: the minimum-image distance is the same FP32
+// expression evaluated in the same order with no fused multiply-add (as
+// src/mic.cuh:20-32 is written), rounding half to even.
+//
+// Three launches, one thread per atom: count the neighbours (d^2 < rc^2, j in
+// the atom's configuration, ascending), write them at the offsets of the
+// exclusive prefix sum of the counts (computed on the host, which needs the
+// list anyway), then find each bond's reverse bond.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "trainset.h"
+
+namespace gpuga {
+
+namespace {
+
+#pragma clang fp contract(off)
+
+struct NLView {
+  int N;
+  float rc2;
+  const float* r;      // [3][N]
+  const float* h;      // [Nc][18] box columns, then the inverse
+  const int* cfg;      // [N] configuration of atom i
+  const int* Na_sum;   // [Nc]
+  const int* Na;       // [Nc]
+};
+
+__device__ __forceinline__ float mic_d2(const float* h, float x, float y, float z)
+{
+  float sx = h[9] * x + h[10] * y + h[11] * z;
+  float sy = h[12] * x + h[13] * y + h[14] * z;
+  float sz = h[15] * x + h[16] * y + h[17] * z;
+  sx -= __builtin_rintf(sx);
+  sy -= __builtin_rintf(sy);
+  sz -= __builtin_rintf(sz);
+  const float rx = h[0] * sx + h[1] * sy + h[2] * sz;
+  const float ry = h[3] * sx + h[4] * sy + h[5] * sz;
+  const float rz = h[6] * sx + h[7] * sy + h[8] * sz;
+  return rx * rx + ry * ry + rz * rz;
+}
+
+// FILL = false: NN[i] = number of neighbours; true: NL[start[i] + k] = the
+// k-th neighbour
+template <bool FILL>
+__global__ void __launch_bounds__(256) nl_pass(NLView v, int* NN, const int* start, int* NL)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.N) return;
+  const int c = v.cfg[i];
+  const float* h = v.h + 18 * (long)c;
+  const int b = v.Na_sum[c], e = b + v.Na[c];
+  const long N = v.N;
+  const float xi = v.r[i], yi = v.r[i + N], zi = v.r[i + 2 * N];
+  int k = 0;
+  int* out = FILL ? NL + start[i] : nullptr;
+  for (int j = b; j < e; ++j) {
+    if (j == i) continue;
+    const float x = v.r[j] - xi, y = v.r[j + N] - yi, z = v.r[j + 2 * N] - zi;
+    if (mic_d2(h, x, y, z) < v.rc2) {
+      if (FILL) out[k] = j;
+      ++k;
+    }
+  }
+  if (!FILL) NN[i] = k;
+}
+
+// rev[b] for bond b = (i, j): the slot of i in j's list, or j's first slot
+// when i is absent
+__global__ void __launch_bounds__(256) nl_reverse(int N, const int* NN, const int* start,
+                                                  const int* NL, int* rev)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int s0 = start[i], n = NN[i];
+  for (int s = 0; s < n; ++s) {
+    const int j = NL[s0 + s];
+    const int t0 = start[j], m = NN[j];
+    int off = 0;
+    for (int u = 0; u < m; ++u)
+      if (NL[t0 + u] == i) {
+        off = u;
+        break;
+      }
+    rev[s0 + s] = t0 + off;
+  }
+}
+
+#pragma clang fp contract(on)
+
+// Device copies of the positions, boxes and configuration index.
+struct NLInput {
+  DeviceBuffer<float> r, h;
+  DeviceBuffer<int> cfg, Na_sum, Na, NN;
+  NLView view{};
+  NLInput(const TrainSet& ts, float cutoff, hipStream_t st)
+  {
+    std::vector<int> c(ts.N);
+    for (int k = 0; k < ts.Nc; ++k)
+      for (int i = ts.Na_sum[k]; i < ts.Na_sum[k] + ts.Na[k]; ++i) c[i] = k;
+    r.upload(ts.r, st);
+    h.upload(ts.h, st);
+    cfg.upload(c, st);
+    Na_sum.upload(ts.Na_sum, st);
+    Na.upload(ts.Na, st);
+    NN.resize(std::max(ts.N, 1));
+    view.N = ts.N;
+    view.rc2 = cutoff * cutoff;
+    view.r = r.get();
+    view.h = h.get();
+    view.cfg = cfg.get();
+    view.Na_sum = Na_sum.get();
+    view.Na = Na.get();
+    // the uploads read host memory that c owns
+    GPUGA_HIP(hipStreamSynchronize(st));
+  }
+  int blocks() const { return (view.N + 255) / 256; }
+  void count(hipStream_t st)
+  {
+    if (view.N == 0) return;
+    nl_pass<false><<<blocks(), 256, 0, st>>>(view, NN.get(), nullptr, nullptr);
+    GPUGA_HIP(hipGetLastError());
+  }
+};
+
+} // namespace
+
+std::vector<int> count_neighbors_device(const TrainSet& ts, float cutoff, hipStream_t st)
+{
+  NLInput in(ts, cutoff, st);
+  in.count(st);
+  std::vector<int> nn(ts.N);
+  in.NN.download(nn.data(), nn.size(), st);
+  GPUGA_HIP(hipStreamSynchronize(st));
+  return nn;
+}
+
+NeighborList build_neighbors_device(const TrainSet& ts, float cutoff, hipStream_t st)
+{
+  NeighborList nl;
+  const int N = ts.N;
+  NLInput in(ts, cutoff, st);
+  in.count(st);
+  nl.NN.resize(N);
+  in.NN.download(nl.NN.data(), N, st);
+  GPUGA_HIP(hipStreamSynchronize(st));
+  nl.start.resize(N);
+  long long total = 0;
+  for (int i = 0; i < N; ++i) {
+    nl.start[i] = (int)total;
+    total += nl.NN[i];
+    nl.max_nn = std::max(nl.max_nn, nl.NN[i]);
+  }
+  if (total > 0x7fffffffLL) fail("neighbour list too large");
+  nl.total = (int)total;
+  nl.NL.resize(total);
+  nl.rev.resize(total);
+  if (total == 0) return nl;
+  DeviceBuffer<int> start, NL, rev;
+  start.upload(nl.start, st);
+  NL.resize(total);
+  rev.resize(total);
+  nl_pass<true><<<in.blocks(), 256, 0, st>>>(in.view, nullptr, start.get(), NL.get());
+  GPUGA_HIP(hipGetLastError());
+  nl_reverse<<<in.blocks(), 256, 0, st>>>(N, in.NN.get(), start.get(), NL.get(), rev.get());
+  GPUGA_HIP(hipGetLastError());
+  NL.download(nl.NL.data(), total, st);
+  rev.download(nl.rev.data(), total, st);
+  GPUGA_HIP(hipStreamSynchronize(st));
+  return nl;
+}
+
+} // namespace gpuga

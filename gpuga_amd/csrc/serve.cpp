@@ -10,10 +10,13 @@ namespace gpuga {
 Server::Server(double budget_bytes, int max_active, const FitnessOptions& opt)
   : budget_(budget_bytes), max_active_(std::max(1, max_active)), opt_(opt)
 {
+  GPUGA_HIP(hipGetDevice(&device_));
 }
 
 Server::~Server()
 {
+  for (auto& j : jobs_)
+    if (j->task.valid()) j->task.wait();
   for (auto& j : jobs_)
     if (j->gaout) std::fclose(j->gaout);
 }
@@ -25,18 +28,25 @@ int Server::submit(const std::string& dir, bool write_outputs)
   j->id = (int)jobs_.size();
   j->dir = dir;
   j->write = write_outputs;
-  try {
-    j->spec = read_potential(dir);
-    j->ts = read_train(dir);
-    j->gp = read_ga(dir);
-    j->est_bytes = Fitness::estimate_bytes(j->spec, j->ts, j->gp.population_size, opt_);
-    queue_.push_back(j->id);
-  } catch (const std::exception& e) {
-    j->state = 3;
-    j->error = e.what();
-  }
+  // the job's files are read (and its device bytes estimated) on a worker
+  Job* jp = j.get();
+  const FitnessOptions opt = opt_;
+  const int dev = device_;
+  j->task = std::async(std::launch::async, [jp, opt, dev] {
+    GPUGA_HIP(hipSetDevice(dev));
+    jp->spec = read_potential(jp->dir);
+    jp->ts = read_train(jp->dir);
+    jp->gp = read_ga(jp->dir);
+    jp->est_bytes = Fitness::estimate_bytes(jp->spec, jp->ts, jp->gp.population_size, opt);
+  });
+  queue_.push_back(j->id);
   jobs_.push_back(std::move(j));
   return jobs_.back()->id;
+}
+
+bool Server::task_done(Job& j) const
+{
+  return j.task.wait_for(std::chrono::seconds(0)) == std::future_status::ready;
 }
 
 int Server::active() const
@@ -55,29 +65,31 @@ int Server::finished() const
 
 void Server::admit(Job& j)
 {
-  try {
-    FitnessOptions o = opt_;
-    j.fit = std::make_unique<Fitness>(j.spec, std::move(j.ts), o);
+  // the job counts as active (slot and memory) from here; its set-up runs
+  // on a worker and the job starts generations once that has finished
+  j.state = 1;
+  j.ready = false;
+  used_ += (double)j.est_bytes;
+  Job* jp = &j;
+  const FitnessOptions opt = opt_;
+  const int dev = device_;
+  j.task = std::async(std::launch::async, [jp, opt, dev] {
+    GPUGA_HIP(hipSetDevice(dev));
+    Job& w = *jp;
+    FitnessOptions o = opt;
+    w.fit = std::make_unique<Fitness>(w.spec, std::move(w.ts), o);
     // deterministic work-item size: the same chunking alone or co-scheduled
-    if (o.chunk_cost <= 0) j.fit->options().chunk_cost = j.fit->heuristic_chunk_cost(j.gp.population_size);
-    // plan and graph at admission: generation latencies then measure
-    // generations only (admission time is kept per job)
-    j.fit->prepare(j.gp.population_size);
-    j.ga = std::make_unique<GA>(j.gp, j.fit->number_of_variables(), j.fit->spec().pmin,
-                                j.fit->spec().pmax);
-    if (j.write) {
-      const std::string path = j.dir + "/ga.out";
-      j.gaout = std::fopen(path.c_str(), "w");
-      if (!j.gaout) fail(strfmt("Failed to open %s!", path.c_str()));
+    if (o.chunk_cost <= 0) w.fit->options().chunk_cost = w.fit->heuristic_chunk_cost(w.gp.population_size);
+    // plan and graph now: generation latencies then measure generations only
+    w.fit->prepare(w.gp.population_size);
+    w.ga = std::make_unique<GA>(w.gp, w.fit->number_of_variables(), w.fit->spec().pmin,
+                                w.fit->spec().pmax);
+    if (w.write) {
+      const std::string path = w.dir + "/ga.out";
+      w.gaout = std::fopen(path.c_str(), "w");
+      if (!w.gaout) fail(strfmt("Failed to open %s!", path.c_str()));
     }
-    j.state = 1;
-    used_ += (double)j.est_bytes;
-    adm_ms_.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
-                                                                j.t_submit)
-                        .count());
-  } catch (const std::exception& e) {
-    retire(j, false, e.what());
-  }
+  });
 }
 
 void Server::launch(Job& j)
@@ -143,7 +155,7 @@ void Server::poll()
   // 1. consume finished generations
   for (auto& jp : jobs_) {
     Job& j = *jp;
-    if (j.state != 1 || !j.inflight) continue;
+    if (j.state != 1 || !j.ready || !j.inflight) continue;
     const hipError_t q = hipStreamQuery(j.fit->stream());
     if (q == hipErrorNotReady) continue;
     if (q != hipSuccess) {
@@ -156,17 +168,45 @@ void Server::poll()
       retire(j, false, e.what());
     }
   }
-  // 2. admit queued jobs while a slot and the memory budget allow
-  while (!queue_.empty() && active() < max_active_) {
+  // 2. jobs whose set-up has finished become ready (admission time: from
+  // submission to ready, waiting for the budget included)
+  for (auto& jp : jobs_) {
+    Job& j = *jp;
+    if (j.state != 1 || j.ready || !task_done(j)) continue;
+    try {
+      j.task.get();
+      j.ready = true;
+      adm_ms_.push_back(
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - j.t_submit)
+          .count());
+    } catch (const std::exception& e) {
+      retire(j, false, e.what());
+    }
+  }
+  // 3. admit queued jobs in order while a slot and the memory budget allow
+  while (!queue_.empty()) {
     Job& j = *jobs_[queue_.front()];
+    if (!j.parsed) {
+      if (!task_done(j)) break; // still reading its files
+      try {
+        j.task.get();
+        j.parsed = true;
+      } catch (const std::exception& e) {
+        queue_.pop_front();
+        j.state = 3;
+        j.error = e.what();
+        continue;
+      }
+    }
+    if (active() >= max_active_) break;
     if (active() > 0 && used_ + (double)j.est_bytes > budget_) break;
     queue_.pop_front();
     admit(j);
   }
-  // 3. start the next generation of every idle active job
+  // 4. start the next generation of every idle ready job
   for (auto& jp : jobs_) {
     Job& j = *jp;
-    if (j.state == 1 && !j.inflight) {
+    if (j.state == 1 && j.ready && !j.inflight) {
       try {
         launch(j);
       } catch (const std::exception& e) {

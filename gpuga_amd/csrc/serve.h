@@ -5,14 +5,18 @@
 // bytes fit a budget (and a slot is free), and each admitted job advances one
 // generation at a time on its own HIP stream, so generations of different
 // jobs overlap on the device.  A finished job releases its memory and the
-// next queued job is admitted (continuous batching).  The work-item size of
-// each job is a deterministic function of its training set, so a job's
-// results are the same whether it runs alone or with others.
+// next queued job is admitted (continuous batching).  Reading a job's files
+// and setting it up (training set on the device, work plan, graph) run on
+// worker threads, so admissions never hold up the generations of running
+// jobs.  The work-item size of each job is a deterministic function of its
+// training set, so a job's results are the same whether it runs alone or
+// with others.
 #pragma once
 
 #include <chrono>
 #include <cstdio>
 #include <deque>
+#include <future>
 #include <memory>
 #include <string>
 #include <vector>
@@ -42,7 +46,9 @@ private:
     int id = 0;
     std::string dir;
     bool write = false;
-    int state = 0; // 0 queued, 1 active, 2 done, 3 failed
+    int state = 0; // 0 queued, 1 active (set-up or running), 2 done, 3 failed
+    bool parsed = false; // files read and device bytes estimated
+    bool ready = false;  // set-up finished: generations can run
     std::string error;
     PotentialSpec spec;
     TrainSet ts;
@@ -56,14 +62,19 @@ private:
     std::chrono::steady_clock::time_point t_launch, t_submit;
     float best = 0.0f;
     std::vector<float> elite;
+    // the worker task (reading, then set-up); last member, so that it is
+    // waited for before the others are destroyed
+    std::future<void> task;
   };
   void admit(Job& j);
+  bool task_done(Job& j) const;
   void launch(Job& j);
   void complete(Job& j);
   void retire(Job& j, bool ok, const std::string& err = "");
 
   double budget_;
   int max_active_;
+  int device_ = 0;
   FitnessOptions opt_;
   std::vector<std::unique_ptr<Job>> jobs_;
   std::deque<int> queue_;

@@ -14,6 +14,8 @@
 
 namespace gpuga {
 
+static void set_box(float* h, const float* t);
+
 namespace {
 
 // Cursor over a whole file with fscanf-like %d / %f / %s conversions: skip
@@ -22,11 +24,12 @@ class Scanner {
 public:
   explicit Scanner(const std::string& path)
   {
-    std::ifstream f(path, std::ios::binary);
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
     if (!f) fail(strfmt("Failed to open %s!", path.c_str()));
-    std::ostringstream ss;
-    ss << f.rdbuf();
-    text_ = ss.str();
+    const std::streamoff n = f.tellg();
+    text_.resize(n > 0 ? (size_t)n : 0);
+    f.seekg(0);
+    if (n > 0 && !f.read(&text_[0], n)) fail(strfmt("Failed to read %s!", path.c_str()));
     p_ = text_.c_str();
   }
   bool get_int(int& v)
@@ -60,6 +63,9 @@ public:
     return true;
   }
 
+  const std::string& text() const { return text_; }
+  size_t offset() const { return (size_t)(p_ - text_.c_str()); }
+
 private:
   void skip()
   {
@@ -68,6 +74,102 @@ private:
   std::string text_;
   const char* p_;
 };
+
+inline bool is_space(char c) { return std::isspace((unsigned char)c) != 0; }
+
+// The body of train.in (after Nc, Nc_force and the Na line) read by 16
+// threads: the text is cut at white space into pieces, each piece counts its
+// tokens, and with the prefix sums each piece knows which field of which
+// configuration its first token is, then converts its tokens in place.
+// Returns false when the text does not hold exactly whole tokens of the
+// expected kinds (a token that does not convert entirely, too few tokens):
+// the caller then reads sequentially, which reports the reference's error.
+bool read_body_parallel(const std::string& text, size_t begin, TrainSet& ts)
+{
+  const int Nc = ts.Nc;
+  const size_t N = ts.N;
+  // tokens per configuration: energy and virial (energy configurations),
+  // box, then type x y z (+ fx fy fz for force configurations) per atom
+  std::vector<long long> t0(Nc + 1, 0);
+  for (int c = 0; c < Nc; ++c)
+    t0[c + 1] = t0[c] + (c >= ts.Nc_force ? 7 : 0) + 9 + (long long)ts.Na[c] * (c < ts.Nc_force ? 7 : 4);
+  const long long need = t0[Nc];
+  const char* s = text.c_str();
+  const size_t end = text.size();
+  // pieces of at least 64 KiB, 16 per thread (parallel_for's grain)
+  const int pieces = (int)std::max<size_t>(1, std::min<size_t>(256, (end - begin) / (1 << 16)));
+  std::vector<size_t> cut(pieces + 1, end);
+  cut[0] = begin;
+  for (int k = 1; k < pieces; ++k) {
+    size_t x = begin + (end - begin) * k / pieces;
+    while (x < end && !is_space(s[x])) ++x; // a piece starts at white space
+    cut[k] = std::max(x, cut[k - 1]);
+  }
+  std::vector<long long> count(pieces, 0);
+  parallel_for(pieces, [&](int k) {
+    long long n = 0;
+    bool in = false;
+    for (size_t x = cut[k]; x < cut[k + 1]; ++x) {
+      const bool sp = is_space(s[x]);
+      n += !sp && !in;
+      in = !sp;
+    }
+    count[k] = n;
+  });
+  std::vector<long long> first(pieces + 1, 0);
+  for (int k = 0; k < pieces; ++k) first[k + 1] = first[k] + count[k];
+  if (first[pieces] < need) return false;
+  std::vector<char> ok(pieces, 1);
+  std::vector<float> box((size_t)Nc * 9);
+  parallel_for(pieces, [&](int k) {
+    long long g = first[k];
+    if (g >= need) return;
+    int c = (int)(std::upper_bound(t0.begin(), t0.end(), g) - t0.begin()) - 1;
+    size_t x = cut[k];
+    while (g < need && g < first[k + 1]) {
+      while (x < cut[k + 1] && is_space(s[x])) ++x;
+      size_t e = x;
+      while (e < end && !is_space(s[e])) ++e;
+      while (g >= t0[c + 1]) ++c;
+      long long f = g - t0[c];
+      const bool energy = c >= ts.Nc_force;
+      char* stop = nullptr;
+      if (energy && f < 7) {
+        const float v = std::strtof(s + x, &stop);
+        if (f == 0) ts.pe_ref[c] = v;
+        else ts.virial_ref[c + (size_t)Nc * (f - 1)] = v;
+      } else {
+        f -= energy ? 7 : 0;
+        if (f < 9) {
+          box[(size_t)c * 9 + f] = std::strtof(s + x, &stop);
+        } else {
+          f -= 9;
+          const int per = energy ? 4 : 7;
+          const size_t i = ts.Na_sum[c] + f / per;
+          const int q = (int)(f % per);
+          if (q == 0) {
+            errno = 0;
+            ts.type[i] = (int)std::strtol(s + x, &stop, 10);
+          } else if (q < 4) {
+            ts.r[i + N * (q - 1)] = std::strtof(s + x, &stop);
+          } else {
+            ts.force_ref[i + N * (q - 4)] = std::strtof(s + x, &stop);
+          }
+        }
+      }
+      if (stop != s + e) {
+        ok[k] = 0;
+        return;
+      }
+      x = e;
+      ++g;
+    }
+  });
+  for (char o : ok)
+    if (!o) return false;
+  for (int c = 0; c < Nc; ++c) set_box(&ts.h[18 * (size_t)c], &box[(size_t)c * 9]);
+  return true;
+}
 
 } // namespace
 
@@ -209,6 +311,10 @@ TrainSet read_train(const std::string& dir)
   ts.type.assign(N, 0);
   ts.r.assign((size_t)N * 3, 0.0f);
   ts.force_ref.assign((size_t)N * 3, 0.0f);
+  if (read_body_parallel(in.text(), in.offset(), ts)) {
+    finish_normalisers(ts);
+    return ts;
+  }
   for (int c = 0; c < Nc; ++c) {
     if (c >= ts.Nc_force) {
       if (!in.get_float(ts.pe_ref[c])) fail(bad);

@@ -10,6 +10,8 @@
 // kernel, src/minimal_tersoff.cu:150-153 and :162-167).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <string>
 #include <vector>
 
@@ -72,6 +74,10 @@ void box_inverse(float* h18);
 void finish_normalisers(TrainSet& ts);
 
 NeighborList build_neighbors(const TrainSet& ts, float cutoff);
+// The same list built on the current HIP device (neighbor.hip), bit-identical.
+NeighborList build_neighbors_device(const TrainSet& ts, float cutoff, hipStream_t st);
+// Its per-atom neighbour counts alone.
+std::vector<int> count_neighbors_device(const TrainSet& ts, float cutoff, hipStream_t st);
 Geometry build_geometry(const TrainSet& ts, const NeighborList& nl, bool triplets);
 
 // Index of the configuration that holds atom i.

@@ -4,6 +4,7 @@ must be the same run alone and run together with other jobs, and the
 memory budget must hold jobs back until memory frees."""
 import os
 import shutil
+import time
 
 import numpy as np
 import pytest
@@ -66,7 +67,17 @@ def test_memory_budget_serialises_jobs(golden, tmp_path):
     dirs = [job_dir(golden, str(tmp_path), "si_diamond", 3, 64, 32, s) for s in (11, 12, 13)]
     s = G.Server(memory_budget_bytes=1.0, max_active=8)  # admits one job at a time
     ids = [s.submit(d) for d in dirs]
-    a, q, f = s.poll()
+    # files are read on worker threads: poll until the first job is admitted
+    seen = []
+    for _ in range(20000):
+        a, q, f = s.poll()
+        seen.append(a)
+        if a > 0 or f > 0:
+            break
+        time.sleep(0.001)
     assert a == 1 and q == 2
-    s.drain()
+    while f < len(ids):
+        a, q, f = s.poll()
+        seen.append(a)
+    assert max(seen) == 1  # never two jobs resident under the budget
     assert all(s.job(i, 9)["state"] == 2 for i in ids)

@@ -61,6 +61,29 @@ def test_synthetic_sets_round_trip_through_both_readers(tmp_path):
     assert ga["population_size"] == 20 and ga["seed"] == 12345678
 
 
+def test_large_train_in_pieces(tmp_path):
+    """A train.in of several MB is read in pieces on several threads; the
+    list built from it equals the oracle's (which reads the file on its
+    own), and a damaged token deep in the file still fails with the
+    reference's message (the sequential reader reports it)."""
+    from gpuga_amd import synth
+    d = synth.tersoff_si(Nc=600, sizes="loguniform", seed=9)
+    synth.write_dir(d, str(tmp_path), ga={"maximum_generation": 5, "population_size": 20,
+                                          "parent_number": 10, "mutation_rate": 0.2})
+    path = os.path.join(str(tmp_path), "train.in")
+    assert os.path.getsize(path) > 2 << 20  # 32 pieces or more: several threads
+    case = O.Case(str(tmp_path))
+    NN, start, NL = G.inspect_neighbors(str(tmp_path))
+    a, b, c = case.neighbors()
+    assert np.array_equal(NN, a) and np.array_equal(start, b) and np.array_equal(NL, c)
+    text = open(path).read()
+    cut = text.rfind(" ", 0, len(text) * 3 // 4)
+    with open(path, "w") as f:
+        f.write(text[:cut] + " 1.0x" + text[cut:])
+    with pytest.raises(G.GpugaError, match="reading error for train.in"):
+        G.inspect_neighbors(str(tmp_path))
+
+
 def write(path, name, text):
     os.makedirs(path, exist_ok=True)
     with open(os.path.join(path, name), "w") as f:
