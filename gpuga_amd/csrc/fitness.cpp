@@ -803,17 +803,19 @@ void Fitness::run_ga(GA& ga, int first_generation, int generations, float* best,
   }
   DeviceGA& dga = *pl.dga;
   dga.upload(ga, first_generation, st);
-  // history chunks of whole batches, so that every batch but the last is full
+  // batches double from one generation up to the staging capacity, so the
+  // device starts after one generation's draws and the host then draws each
+  // batch while the device runs the one before
   const int B = dga.batch();
-  const int H = std::max(B, dga.history_cap() / B * B);
+  const int H = dga.history_cap();
   std::vector<float> h((size_t)H * (1 + V));
   int g = 0; // generations queued
   while (g < generations) {
     const int n = std::min(H, generations - g);
     if (g > 0) dga.restart_history(first_generation + g, st);
     if (timing_) GPUGA_HIP(hipEventRecord(ev0_, st));
-    for (int b = 0; b < n; b += B) {
-      const int nb = std::min(B, n - b);
+    for (int b = 0, ramp = 1, nb = 0; b < n; b += nb, ramp = std::min(2 * ramp, B)) {
+      nb = std::min({ramp, B, n - b});
       dga.stage(ga, first_generation + g + b, nb, st);
       // the batch as runs of 2^j generations, each one graph (no launch
       // gap between the generations inside a graph)

@@ -352,7 +352,7 @@ DeviceGA::DeviceGA(const GA& ga, float* genes0, int history_cap, hipStream_t st,
   const size_t budget = (size_t)1 << 22; // words per batch
   const size_t most = max_batch > 0 ? std::min(max_batch, kGaMaxBatch) : kGaMaxBatch;
   const int batch = (int)std::max<size_t>(1, std::min<size_t>(most, budget / rec_max));
-  stage_words_ = (size_t)batch + (size_t)batch * rec_max;
+  stage_words_ = 1 + (size_t)batch + (size_t)batch * rec_max;
   if (stage_words_ > (size_t)INT_MAX) fail("device GA: draw records too large");
   draws_.resize(stage_words_);
   for (int k = 0; k < 2; ++k) {
@@ -422,16 +422,16 @@ void DeviceGA::stage(GA& ga, int generation, int n, hipStream_t st)
 {
   const int B = d_.batch;
   if (n < 1 || n > B) fail(strfmt("device GA: %d records staged, batches hold 1-%d", n, B));
-  if (staged_ % B != 0) fail("device GA: records after a partial batch");
   if (generation < 0) fail("generation numbers start at 0");
   const int k = next_;
   next_ ^= 1;
   // the staging area's previous copy has run (it was queued two batches ago)
   if (pending_[k]) GPUGA_HIP(hipEventSynchronize(copied_[k]));
   int* h = host_[k].get();
-  size_t w = (size_t)B;
+  h[0] = (int)staged_; // the step that reads the first record
+  size_t w = (size_t)B + 1;
   for (int i = 0; i < n; ++i) {
-    h[i] = (int)w;
+    h[1 + i] = (int)w;
     cross_.clear();
     idx_.clear();
     val_.clear();

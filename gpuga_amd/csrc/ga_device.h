@@ -15,13 +15,14 @@ class GA;
 //
 // Draw records: the random decisions of one generation, drawn on the host
 // (GA::draw_crossover / GA::draw_mutation) and staged in a batch buffer of
-// `batch` records: words [0, batch) are record offsets (in words, from the
-// buffer start), a record is
+// at most `batch` records: word 0 is the step (counted from the upload) that
+// reads the first record, words [1, batch] the record offsets (in words,
+// from the buffer start), and a record is
 //   [0]                      n = mutations after removing overwritten ones
 //   [1, 1 + 3 * pairs)       crossover: parent rank p1, p2, crossover point
 //   [.., + n)                gene index of each mutation
 //   [.., + n)                its value (float bits)
-// Step s after the upload reads record s mod batch.
+// Step s reads record s - word 0 of the buffer staged before it.
 struct GaDev {
   int P, V, parents, children;
   int rows;                // population rows per workgroup
@@ -73,8 +74,9 @@ public:
   int batch() const { return d_.batch; }
   // Draw the records of `n` <= batch() generations from generation
   // `generation` on the host GA (advancing its random stream as the host
-  // steps would) and queue their copy to the device on st.  The steps that
-  // read them must be queued after this call and before the next stage().
+  // steps would) and queue their copy to the device on st.  The n steps
+  // that read them must be queued after this call and before the next
+  // stage().
   void stage(GA& ga, int generation, int n, hipStream_t st);
   // one step reading population buffer k and writing buffer 1 - k, on
   // fitness (device pointer); graph-capturable

@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(GA_THREADS) ga_step(GaDev d)
   float* tile = reinterpret_cast<float*>(cross + 3 * pairs);
   __shared__ int s_nan, s_gen, s_gen0;
   __shared__ unsigned s_step;
-  __shared__ int s_off[kGaMaxBatch];
+  __shared__ int s_off[kGaMaxBatch + 1];
   const int t = threadIdx.x, T = blockDim.x;
 
   // counters, the batch's record offsets and the fitness in one round trip
@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(GA_THREADS) ga_step(GaDev d)
     s_gen = aload(d.generation);
     s_gen0 = aload(d.history_gen0);
   }
-  if (t < d.batch) s_off[t] = d.draws[t];
+  if (t <= d.batch) s_off[t] = d.draws[t];
   for (int i = t; i < Ppad; i += T) {
     // padding sorts after every entry: +inf with an index >= P
     const float f = i < P ? d.fitness[i] : __builtin_inff();
@@ -89,7 +89,11 @@ __global__ void __launch_bounds__(GA_THREADS) ga_step(GaDev d)
   }
   __syncthreads();
   const unsigned step = s_step;
-  const int* rec = d.draws + s_off[step % (unsigned)d.batch];
+  // word 0: the step of the batch's first record; then the offsets (the
+  // host stages exactly the steps it queues; the clamp only keeps a
+  // misuse inside the buffer)
+  const int r = min(max((int)(step - (unsigned)s_off[0]), 0), d.batch - 1);
+  const int* rec = d.draws + s_off[1 + r];
 
   // ---- sort_population (src/ga.cu:105-136), with the record's crossover
   // words copied to LDS beside it
