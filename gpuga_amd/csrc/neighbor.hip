@@ -51,7 +51,11 @@ __global__ void __launch_bounds__(256) nl_pass(NLView v, int* NN, const int* sta
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= v.N) return;
   const int c = v.cfg[i];
-  const float* h = v.h + 18 * (long)c;
+  // the box in registers (read through the pointer, it is loaded again for
+  // every pair)
+  float h[18];
+#pragma unroll
+  for (int k = 0; k < 18; ++k) h[k] = v.h[18 * (long)c + k];
   const int b = v.Na_sum[c], e = b + v.Na[c];
   const long N = v.N;
   const float xi = v.r[i], yi = v.r[i + N], zi = v.r[i + 2 * N];
