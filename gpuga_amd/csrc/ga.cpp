@@ -28,52 +28,21 @@ void MT19937::set_state(const uint32_t* s, int idx)
   idx_ = idx;
 }
 
-uint32_t MT19937::operator()()
+void MT19937::twist()
 {
-  if (idx_ >= 624) {
-    static const uint32_t mag[2] = {0u, 0x9908b0dfu};
-    int i = 0;
-    for (; i < 624 - 397; ++i) {
-      const uint32_t y = (mt_[i] & 0x80000000u) | (mt_[i + 1] & 0x7fffffffu);
-      mt_[i] = mt_[i + 397] ^ (y >> 1) ^ mag[y & 1u];
-    }
-    for (; i < 623; ++i) {
-      const uint32_t y = (mt_[i] & 0x80000000u) | (mt_[i + 1] & 0x7fffffffu);
-      mt_[i] = mt_[i + 397 - 624] ^ (y >> 1) ^ mag[y & 1u];
-    }
-    const uint32_t y = (mt_[623] & 0x80000000u) | (mt_[0] & 0x7fffffffu);
-    mt_[623] = mt_[396] ^ (y >> 1) ^ mag[y & 1u];
-    idx_ = 0;
+  // the state update of std::mt19937 (C++ [rand.eng.mers]), branch-free
+  int i = 0;
+  for (; i < 624 - 397; ++i) {
+    const uint32_t y = (mt_[i] & 0x80000000u) | (mt_[i + 1] & 0x7fffffffu);
+    mt_[i] = mt_[i + 397] ^ (y >> 1) ^ ((0u - (y & 1u)) & 0x9908b0dfu);
   }
-  uint32_t y = mt_[idx_++];
-  y ^= y >> 11;
-  y ^= (y << 7) & 0x9d2c5680u;
-  y ^= (y << 15) & 0xefc60000u;
-  y ^= y >> 18;
-  return y;
-}
-
-float MT19937::canonical()
-{
-  const float x = (float)(*this)();
-  float r = x / 4294967296.0f;
-  if (r >= 1.0f) r = std::nextafter(1.0f, 0.0f);
-  return r;
-}
-
-int MT19937::uniform_int(int a, int b)
-{
-  const uint32_t range = (uint32_t)b - (uint32_t)a + 1u;
-  uint64_t m = (uint64_t)(*this)() * range;
-  uint32_t low = (uint32_t)m;
-  if (low < range) {
-    const uint32_t threshold = (uint32_t)(0u - range) % range;
-    while (low < threshold) {
-      m = (uint64_t)(*this)() * range;
-      low = (uint32_t)m;
-    }
+  for (; i < 623; ++i) {
+    const uint32_t y = (mt_[i] & 0x80000000u) | (mt_[i + 1] & 0x7fffffffu);
+    mt_[i] = mt_[i + 397 - 624] ^ (y >> 1) ^ ((0u - (y & 1u)) & 0x9908b0dfu);
   }
-  return (int)(uint32_t)(m >> 32) + a;
+  const uint32_t y = (mt_[623] & 0x80000000u) | (mt_[0] & 0x7fffffffu);
+  mt_[623] = mt_[396] ^ (y >> 1) ^ ((0u - (y & 1u)) & 0x9908b0dfu);
+  idx_ = 0;
 }
 
 // --------------------------------------------------------------- ga.in
@@ -348,7 +317,11 @@ DeviceGA::DeviceGA(const GA& ga, float* genes0, int history_cap, hipStream_t st,
   // with the generation) and at most one per gene outside the best row
   pairs_ = (P - p.parent_number) / 2;
   max_mut_ = std::min(ga.mutation_count(0), (int)(PV - (size_t)V));
-  const size_t rec_max = 1 + 3 * (size_t)pairs_ + 2 * (size_t)max_mut_;
+  int rows = 0;
+  size_t lds = 0;
+  ga_step_shape(P, V, &rows, &lds);
+  groups_ = (P + rows - 1) / rows; // workgroups of a step
+  const size_t rec_max = 1 + 3 * (size_t)pairs_ + (size_t)groups_ + 1 + 2 * (size_t)max_mut_;
   const size_t budget = (size_t)1 << 22; // words per batch
   const size_t most = max_batch > 0 ? std::min(max_batch, kGaMaxBatch) : kGaMaxBatch;
   const int batch = (int)std::max<size_t>(1, std::min<size_t>(most, budget / rec_max));
@@ -360,9 +333,6 @@ DeviceGA::DeviceGA(const GA& ga, float* genes0, int history_cap, hipStream_t st,
     GPUGA_HIP(hipEventCreateWithFlags(&copied_[k], hipEventDisableTiming));
   }
   seen_.assign(PV, 0);
-  int rows = 0;
-  size_t lds = 0;
-  ga_step_shape(P, V, &rows, &lds);
   d_.P = P;
   d_.V = V;
   d_.parents = p.parent_number;
@@ -439,26 +409,60 @@ void DeviceGA::stage(GA& ga, int generation, int n, hipStream_t st)
     const int count = ga.draw_mutation(generation + i, idx_, val_);
     int* r = h + w;
     std::memcpy(r + 1, cross_.data(), sizeof(int) * 3 * (size_t)pairs_);
-    // the last mutation of a gene is the one that stays: walk backwards and
-    // keep each gene's first sighting
-    if (++seen_tag_ == INT_MAX) {
-      std::fill(seen_.begin(), seen_.end(), 0);
-      seen_tag_ = 1;
-    }
-    int* ri = r + 1 + 3 * pairs_;
-    keep_.clear();
-    for (int j = count - 1; j >= 0; --j) {
+    // Grouped by the workgroup whose rows they fall in (each workgroup reads
+    // its own range), and within a group the last mutation of a gene is the
+    // one that stays.  Three cache-friendly passes: group counts, a stable
+    // scatter by group, then per group a backward walk that keeps each gene's
+    // first sighting (a stamp per gene of the group's rows).
+    int* bo = r + 1 + 3 * pairs_; // [groups + 1]
+    const long span = (long)d_.rows * d_.V; // genes per workgroup
+    const double inv_span = 1.0 / (double)span;
+    fill_.assign(groups_ + 1, 0);
+    kgrp_.resize(count);
+    for (int j = 0; j < count; ++j) {
       const int g = idx_[j];
-      if (seen_[g] == seen_tag_) continue;
-      seen_[g] = seen_tag_;
-      ri[keep_.size()] = g;
-      keep_.push_back(val_[j]);
+      // g / span by a multiplication, corrected to the exact quotient
+      int q = groups_ == 1 ? 0 : (int)((double)g * inv_span);
+      if ((long)q * span > g) --q;
+      else if ((long)(q + 1) * span <= g) ++q;
+      kgrp_[j] = q;
+      ++fill_[q + 1];
     }
-    const int m = (int)keep_.size();
-    if (m > max_mut_) fail("device GA: more mutations than the staging area holds");
-    std::memcpy(ri + m, keep_.data(), sizeof(float) * (size_t)m);
+    for (int q = 0; q < groups_; ++q) fill_[q + 1] += fill_[q];
+    kidx_.resize(count);
+    keep_.resize(count);
+    for (int j = 0; j < count; ++j) {
+      const int at = fill_[kgrp_[j]]++;
+      kidx_[at] = idx_[j];
+      keep_[at] = val_[j];
+    }
+    // fill_[q] is now the end of group q's entries; the kept indices go
+    // straight into the record, their values into val_ (free by now), since
+    // the values follow all the indices
+    int* ri = bo + groups_ + 1;
+    int m = 0;
+    for (int q = 0; q < groups_; ++q) {
+      bo[q] = m;
+      const int begin = q == 0 ? 0 : fill_[q - 1], end = fill_[q];
+      if (++seen_tag_ == INT_MAX) {
+        std::fill(seen_.begin(), seen_.end(), 0);
+        seen_tag_ = 1;
+      }
+      const long base = (long)q * span;
+      for (int j = end - 1; j >= begin; --j) {
+        const int local = (int)(kidx_[j] - base);
+        if (seen_[local] == seen_tag_) continue;
+        seen_[local] = seen_tag_;
+        if (m >= max_mut_) fail("device GA: more mutations than the staging area holds");
+        ri[m] = kidx_[j];
+        val_[m] = keep_[j];
+        ++m;
+      }
+    }
+    bo[groups_] = m;
+    std::memcpy(ri + m, val_.data(), sizeof(float) * (size_t)m);
     r[0] = m;
-    w += 1 + 3 * (size_t)pairs_ + 2 * (size_t)m;
+    w += 1 + 3 * (size_t)pairs_ + (size_t)groups_ + 1 + 2 * (size_t)m;
   }
   GPUGA_HIP(hipMemcpyAsync(draws_.get(), h, sizeof(int) * w, hipMemcpyHostToDevice, st));
   GPUGA_HIP(hipEventRecord(copied_[k], st));

@@ -19,19 +19,47 @@ class MT19937 {
 public:
   explicit MT19937(uint32_t seed = 5489u) { seed_with(seed); }
   void seed_with(uint32_t seed);
-  uint32_t operator()();
+  // (inline: the GA draws hundreds of thousands per generation)
+  uint32_t operator()()
+  {
+    if (idx_ >= 624) twist();
+    uint32_t y = mt_[idx_++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+  }
   // uniform_real_distribution<float>(0, 1) of libstdc++: generate_canonical
   // with one 32-bit draw, (float)x / 2^32, results >= 1 replaced by the float
   // just below 1.
-  float canonical();
+  float canonical()
+  {
+    const float r = (float)(*this)() * 2.3283064365386963e-10f; // exact: 2^-32
+    return r >= 1.0f ? 0.99999994f : r;                          // nextafter(1, 0)
+  }
   // uniform_int_distribution<int>(a, b) of libstdc++ >= 11: Lemire's
   // nearly-divisionless method on a 64-bit product (ACM TOMACS 29(1), 2019).
-  int uniform_int(int a, int b);
+  int uniform_int(int a, int b)
+  {
+    const uint32_t range = (uint32_t)b - (uint32_t)a + 1u;
+    uint64_t m = (uint64_t)(*this)() * range;
+    uint32_t low = (uint32_t)m;
+    if (low < range) {
+      const uint32_t threshold = (uint32_t)(0u - range) % range;
+      while (low < threshold) {
+        m = (uint64_t)(*this)() * range;
+        low = (uint32_t)m;
+      }
+    }
+    return (int)(uint32_t)(m >> 32) + a;
+  }
   const uint32_t* state() const { return mt_; }
   int index() const { return idx_; }
   void set_state(const uint32_t* s, int idx);
 
 private:
+  void twist();
   uint32_t mt_[624];
   int idx_;
 };

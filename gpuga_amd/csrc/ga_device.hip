@@ -174,11 +174,13 @@ __global__ void __launch_bounds__(GA_THREADS) ga_step(GaDev d)
   const int nel = (r1 - r0) * V;
   const long e0 = (long)r0 * V;
   const int nm = rec[0];
-  const int* mi = rec + 1 + 3 * pairs;
+  const int* bo = rec + 1 + 3 * pairs; // this workgroup's mutations: [bo[g], bo[g + 1])
+  const int* mi = bo + gridDim.x + 1;
   const float* mv = reinterpret_cast<const float*>(mi + nm);
+  const int j0 = bo[blockIdx.x], j1 = bo[blockIdx.x + 1];
   // this thread's first mutation, loaded while the rows are gathered
-  const int mi0 = t < nm ? mi[t] : -1;
-  const float mv0 = t < nm ? mv[t] : 0.0f;
+  const int mi0 = j0 + t < j1 ? mi[j0 + t] : -1;
+  const float mv0 = j0 + t < j1 ? mv[j0 + t] : 0.0f;
   for (int e = t; e < nel; e += T) {
     const int n = r0 + e / V, m = e % V;
     int src;
@@ -197,7 +199,7 @@ __global__ void __launch_bounds__(GA_THREADS) ga_step(GaDev d)
     const long i = (long)mi0 - e0;
     if (i >= 0 && i < nel) tile[i] = mv0;
   }
-  for (int j = t + T; j < nm; j += T) {
+  for (int j = j0 + t + T; j < j1; j += T) {
     const long i = (long)mi[j] - e0;
     if (i >= 0 && i < nel) tile[i] = mv[j];
   }

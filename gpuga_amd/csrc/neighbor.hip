@@ -73,7 +73,8 @@ __global__ void __launch_bounds__(256) nl_pass(NLView v, int* NN, const int* sta
 }
 
 // rev[b] for bond b = (i, j): the slot of i in j's list, or j's first slot
-// when i is absent
+// when i is absent (lists are ascending and without repeats, so a binary
+// search finds the slot the linear search of the host build finds)
 __global__ void __launch_bounds__(256) nl_reverse(int N, const int* NN, const int* start,
                                                   const int* NL, int* rev)
 {
@@ -82,13 +83,14 @@ __global__ void __launch_bounds__(256) nl_reverse(int N, const int* NN, const in
   const int s0 = start[i], n = NN[i];
   for (int s = 0; s < n; ++s) {
     const int j = NL[s0 + s];
-    const int t0 = start[j], m = NN[j];
-    int off = 0;
-    for (int u = 0; u < m; ++u)
-      if (NL[t0 + u] == i) {
-        off = u;
-        break;
-      }
+    const int t0 = start[j];
+    int lo = 0, hi = NN[j]; // first slot with NL >= i
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (NL[t0 + mid] < i) lo = mid + 1;
+      else hi = mid;
+    }
+    const int off = lo < NN[j] && NL[t0 + lo] == i ? lo : 0;
     rev[s0 + s] = t0 + off;
   }
 }
