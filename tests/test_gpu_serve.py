@@ -81,3 +81,24 @@ def test_memory_budget_serialises_jobs(golden, tmp_path):
         seen.append(a)
     assert max(seen) == 1  # never two jobs resident under the budget
     assert all(s.job(i, 9)["state"] == 2 for i in ids)
+
+
+def test_failed_jobs_do_not_stop_the_others(golden, tmp_path):
+    """A job whose files cannot be read fails on its worker with the
+    reference's message; the other jobs are served to the end."""
+    import gpuga_amd as G
+    print_case("serve failure")
+    good = job_dir(golden, str(tmp_path), "si_diamond", 3, 64, 32, 21)
+    bad = os.path.join(str(tmp_path), "missing")
+    broken = job_dir(golden, str(tmp_path), "gaas", 3, 40, 20, 22)
+    with open(os.path.join(broken, "ga.in"), "w") as f:
+        f.write("population_size 10\n")
+    s = G.Server(memory_budget_bytes=64e9, max_active=8)
+    ids = [s.submit(d) for d in (bad, good, broken)]
+    s.drain()
+    with pytest.raises(G.GpugaError, match="Failed to open"):
+        s.job(ids[0], 9)
+    assert s.job(ids[1], 9)["state"] == 2 and s.job(ids[1], 9)["generations"] == 3
+    with pytest.raises(G.GpugaError, match="population_size should >= 20"):
+        s.job(ids[2], 9)
+    assert len(s.admissions()) == 1
