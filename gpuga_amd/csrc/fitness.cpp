@@ -752,13 +752,32 @@ void Fitness::launch_generation(PopPlan& pl, hipStream_t st)
   }
 }
 
+namespace {
+
+// The launches of `body` on st as a graph, instantiated.  A failure inside
+// ends the capture first, so the stream is usable again.
+template <class F>
+void captured(hipStream_t st, F&& body, hipGraph_t* graph, hipGraphExec_t* exec)
+{
+  GPUGA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  GPUGA_HIP(hipStreamEndCapture(st, graph));
+  GPUGA_HIP(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+}
+
+} // namespace
+
 void Fitness::capture(PopPlan& pl, hipStream_t st)
 {
   if (pl.exec) return;
-  GPUGA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  launch_population(pl, st);
-  GPUGA_HIP(hipStreamEndCapture(st, &pl.graph));
-  GPUGA_HIP(hipGraphInstantiate(&pl.exec, pl.graph, nullptr, nullptr, 0));
+  captured(st, [&] { launch_population(pl, st); }, &pl.graph, &pl.exec);
 }
 
 void Fitness::prepare(int P)
@@ -774,14 +793,13 @@ void Fitness::capture_generations(PopPlan& pl, int k, int n, hipGraph_t* graph,
                                   hipGraphExec_t* exec)
 {
   hipStream_t st = stream_;
-  GPUGA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  for (int i = 0; i < n; ++i) {
-    const int b = (k + i) & 1;
-    launch_population(pl, st, pl.dga->genes(b));
-    pl.dga->step(b, pl.ctl.fitness, st);
-  }
-  GPUGA_HIP(hipStreamEndCapture(st, graph));
-  GPUGA_HIP(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+  captured(st, [&] {
+    for (int i = 0; i < n; ++i) {
+      const int b = (k + i) & 1;
+      launch_population(pl, st, pl.dga->genes(b));
+      pl.dga->step(b, pl.ctl.fitness, st);
+    }
+  }, graph, exec);
 }
 
 void Fitness::run_ga(GA& ga, int first_generation, int generations, float* best, float* elite)

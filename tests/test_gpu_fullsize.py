@@ -1,0 +1,68 @@
+"""The headline workload at its full size (population 200, 1000 diamond-Si
+configurations of 8-512 atoms, 120 200 atoms): the population launch
+against the FP64 oracle for a sample of individuals (the oracle evaluates
+one individual over the whole set in about 0.1 s), against the
+per-individual launches for another sample, and bit-identical over
+repeated launches and over the GA run's generations (replayed graphs).
+Tolerance: fitness and terms 1e-5 relative (FP32 bond arithmetic, FP64
+sums)."""
+import numpy as np
+import pytest
+
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def headline():
+    import gpuga_amd
+    from gpuga_amd import synth
+    d = synth.tersoff_si(Nc=1000, sizes="loguniform", seed=2)
+    lab = gpuga_amd.Fitness(arrays=d)
+    d = synth.label(d, lab.find_force)
+    lab.close()
+    fit = gpuga_amd.Fitness(arrays=d)
+    yield d, fit
+    fit.close()
+
+
+def test_full_size_population_matches_oracle(headline):
+    from oracle import oracle as O
+    d, fit = headline
+    print_case("full size vs oracle", fit.N)
+    assert fit.N == 120200 and fit.Nc == 1000
+    genes = np.random.default_rng(77).random((200, fit.V), dtype=np.float32)
+    f, terms = fit.compute(genes, terms=True)
+    case = O.Case(potential=dict(potential_type=1, cutoff=3.0, weights=(0.05, 0.15, 0.8),
+                                 pmin=d["pmin"], pmax=d["pmax"]), arrays=d)
+    sel = [0, 63, 64, 127, 191, 192, 199]  # every lane block, the packed one included
+    ref, ref_terms = case.fitness(genes[sel], threads=16)
+    np.testing.assert_allclose(terms[sel], ref_terms, rtol=1e-5)
+    np.testing.assert_allclose(f[sel], ref, rtol=1e-5)
+    again = fit.compute(genes)
+    assert np.array_equal(f, again)
+
+
+def test_full_size_population_equals_per_individual_launches(headline):
+    _, fit = headline
+    print_case("full size vs per-individual")
+    genes = np.random.default_rng(78).random((200, fit.V), dtype=np.float32)
+    f, terms = fit.compute(genes, terms=True)
+    sel = [5, 100, 198]
+    fp, tp = fit.compute(genes[sel], terms=True, plain=True)
+    np.testing.assert_allclose(tp, terms[sel], rtol=1e-5)
+    np.testing.assert_allclose(fp, f[sel], rtol=1e-5)
+
+
+def test_full_size_ga_run_equals_host_loop(headline):
+    import gpuga_amd as G
+    _, fit = headline
+    print_case("full size run_ga vs host loop")
+    gens = 6
+    host = G.GA(200, 100, fit.V, maximum_generation=gens, mutation_rate=0.2, seed=12345678)
+    dev = G.GA(200, 100, fit.V, maximum_generation=gens, mutation_rate=0.2, seed=12345678)
+    best_h = [host.step(g, fit.compute(host.population))[0] for g in range(gens)]
+    best_d, _ = fit.run_ga(dev, 0, gens)
+    assert np.array_equal(np.array(best_h, np.float32), best_d)
+    assert np.array_equal(host.population, dev.population)
