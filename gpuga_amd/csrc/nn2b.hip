@@ -99,16 +99,42 @@ struct NN2BPop {
   // (the reference's force expression, src/nn2b.cu:143-148), with U and U'
   // of this lane's individual interpolated from the rows (p, p', p'') of
   // the cell's two grid points.
-  __device__ static __forceinline__ void pair(const float* __restrict__ tk, const float4& w,
-                                              const float4& c, float& p2, float& G)
+  struct Rows {
+    float p0, d0, s0, p1, d1, s1;
+  };
+  __device__ static __forceinline__ Rows rows(const float* __restrict__ tk, const float4& c)
   {
     const float* r0 = tk + (size_t)__float_as_int(c.z) * 3 * WAVE;
-    const float p0 = r0[0], d0 = r0[WAVE], s0 = r0[2 * WAVE];
-    const float p1 = r0[3 * WAVE], d1 = r0[4 * WAVE], s1 = r0[5 * WAVE];
-    const float p = fmaf(w.x, p0, fmaf(w.y, d0, fmaf(w.z, p1, w.w * d1)));
-    const float pd = fmaf(w.x, d0, fmaf(w.y, s0, fmaf(w.z, d1, w.w * s1)));
+    return Rows{r0[0], r0[WAVE], r0[2 * WAVE], r0[3 * WAVE], r0[4 * WAVE], r0[5 * WAVE]};
+  }
+  __device__ static __forceinline__ void pair(const Rows& r, const float4& w, const float4& c,
+                                              float& p2, float& G)
+  {
+    const float p = fmaf(w.x, r.p0, fmaf(w.y, r.d0, fmaf(w.z, r.p1, w.w * r.d1)));
+    const float pd = fmaf(w.x, r.d0, fmaf(w.y, r.s0, fmaf(w.z, r.d1, w.w * r.s1)));
     p2 = p * c.x;
     G = fmaf(pd, c.x, p2 * c.y);
+  }
+  // Pairs [k0, n) of an atom's table in steps of two: both pairs' table
+  // rows are loaded before either is evaluated, so the lookups (a gather
+  // that depends on the scalar load of the cell) are waited for once per
+  // two pairs.
+  template <class F>
+  __device__ static __forceinline__ void each_pair(const float* __restrict__ tk,
+                                                   const float4* tab, int n, F&& f)
+  {
+    int k = 0;
+    for (; k + 2 <= n; k += 2) {
+      const float4 u0 = ldu(tab, 3 * k), w0 = ldu(tab, 3 * k + 1), c0 = ldu(tab, 3 * k + 2);
+      const float4 u1 = ldu(tab, 3 * k + 3), w1 = ldu(tab, 3 * k + 4), c1 = ldu(tab, 3 * k + 5);
+      const Rows r0 = rows(tk, c0), r1 = rows(tk, c1);
+      f(u0, r0, w0, c0);
+      f(u1, r1, w1, c1);
+    }
+    if (k < n) {
+      const float4 u = ldu(tab, 3 * k), w = ldu(tab, 3 * k + 1), c = ldu(tab, 3 * k + 2);
+      f(u, rows(tk, c), w, c);
+    }
   }
 
   __device__ static void chunk(const DevSet& s, const Lane&, const WorkItem& it, const PopCtl&,
@@ -122,14 +148,14 @@ struct NN2BPop {
         const int4 h = ldu(s.pair_hdr, i);
         const float4* tab = s.pair_tab + 3 * (long)h.x;
         float fx = 0, fy = 0, fz = 0;
-        for (int k = 0; k < h.y; ++k) {
-          const float4 u = ldu(tab, 3 * k), w = ldu(tab, 3 * k + 1), c = ldu(tab, 3 * k + 2);
+        each_pair(tk, tab, h.y, [&](const float4& u, const Rows& r, const float4& w,
+                                    const float4& c) {
           float p2, G;
-          pair(tk, w, c, p2, G);
+          pair(r, w, c, p2, G);
           fx = fmaf(u.x, G, fx);
           fy = fmaf(u.y, G, fy);
           fz = fmaf(u.z, G, fz);
-        }
+        });
         const float dx = fx - ldu(s.force_ref, i), dy = fy - ldu(s.force_ref, i + N),
                     dz = fz - ldu(s.force_ref, i + 2 * N);
         acc.v[7] += (double)dx * dx + (double)dy * dy + (double)dz * dz;
@@ -140,10 +166,10 @@ struct NN2BPop {
         const int4 h = ldu(s.pair_hdr, i);
         const float4* tab = s.pair_tab + 3 * (long)h.x;
         float e = 0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wyz = 0, wzx = 0;
-        for (int k = 0; k < h.y; ++k) {
-          const float4 u = ldu(tab, 3 * k), w = ldu(tab, 3 * k + 1), c = ldu(tab, 3 * k + 2);
+        each_pair(tk, tab, h.y, [&](const float4& u, const Rows& r, const float4& w,
+                                    const float4& c) {
           float p2, G;
-          pair(tk, w, c, p2, G);
+          pair(r, w, c, p2, G);
           const float t = u.w * G;
           const float tx = u.x * t, ty = u.y * t, tz = u.z * t;
           e += p2;
@@ -153,7 +179,7 @@ struct NN2BPop {
           wxy = fmaf(-u.x, ty, wxy);
           wyz = fmaf(-u.y, tz, wyz);
           wzx = fmaf(-u.z, tx, wzx);
-        }
+        });
         acc.v[0] += e;
         acc.v[1] += wxx;
         acc.v[2] += wyy;
