@@ -115,15 +115,24 @@ struct NN2BPop {
     p2 = p * c.x;
     G = fmaf(pd, c.x, p2 * c.y);
   }
-  // Pairs [k0, n) of an atom's table in steps of two: both pairs' table
-  // rows are loaded before either is evaluated, so the lookups (a gather
-  // that depends on the scalar load of the cell) are waited for once per
-  // two pairs.
+  // The pairs of an atom's table, four (then two, then one) at a time: the
+  // pairs' table rows are all loaded before any is evaluated, so the
+  // lookups (a gather that depends on the scalar load of the cell) are
+  // waited for once per step rather than once per pair.
   template <class F>
   __device__ static __forceinline__ void each_pair(const float* __restrict__ tk,
                                                    const float4* tab, int n, F&& f)
   {
     int k = 0;
+    for (; k + 4 <= n; k += 4) {
+      const float4 c0 = ldu(tab, 3 * k + 2), c1 = ldu(tab, 3 * k + 5);
+      const float4 c2 = ldu(tab, 3 * k + 8), c3 = ldu(tab, 3 * k + 11);
+      const Rows r0 = rows(tk, c0), r1 = rows(tk, c1), r2 = rows(tk, c2), r3 = rows(tk, c3);
+      f(ldu(tab, 3 * k), r0, ldu(tab, 3 * k + 1), c0);
+      f(ldu(tab, 3 * k + 3), r1, ldu(tab, 3 * k + 4), c1);
+      f(ldu(tab, 3 * k + 6), r2, ldu(tab, 3 * k + 7), c2);
+      f(ldu(tab, 3 * k + 9), r3, ldu(tab, 3 * k + 10), c3);
+    }
     for (; k + 2 <= n; k += 2) {
       const float4 u0 = ldu(tab, 3 * k), w0 = ldu(tab, 3 * k + 1), c0 = ldu(tab, 3 * k + 2);
       const float4 u1 = ldu(tab, 3 * k + 3), w1 = ldu(tab, 3 * k + 4), c1 = ldu(tab, 3 * k + 5);
