@@ -152,11 +152,22 @@ struct RIPop {
 
   // The pairs of type T of one atom: F_i += u g (force configurations) or
   // E += U, W -= u (x) u d g over the unordered pairs (the others).  The
-  // geometry is wave-uniform (scalar loads); two pairs per step.
+  // geometry is wave-uniform (scalar loads); four pairs per step (their
+  // loads issue together and are waited for once), then two, then one.
   template <int T, bool FORCE>
   __device__ static void pairs(const RILane& q, const float4* tab, int n, float* a)
   {
     int k = 0;
+    for (; k + 4 <= n; k += 4) {
+      const float4 u0 = ldu(tab, 2 * k), f0 = ldu(tab, 2 * k + 1);
+      const float4 u1 = ldu(tab, 2 * k + 2), f1 = ldu(tab, 2 * k + 3);
+      const float4 u2 = ldu(tab, 2 * k + 4), f2 = ldu(tab, 2 * k + 5);
+      const float4 u3 = ldu(tab, 2 * k + 6), f3 = ldu(tab, 2 * k + 7);
+      one<T, FORCE>(q, u0, f0, a);
+      one<T, FORCE>(q, u1, f1, a);
+      one<T, FORCE>(q, u2, f2, a);
+      one<T, FORCE>(q, u3, f3, a);
+    }
     for (; k + 2 <= n; k += 2) {
       const float4 u0 = ldu(tab, 2 * k), f0 = ldu(tab, 2 * k + 1);
       const float4 u1 = ldu(tab, 2 * k + 2), f1 = ldu(tab, 2 * k + 3);
