@@ -42,6 +42,10 @@ class Options(C.Structure):
                 ("ga_batch", C.c_int)]
 
 
+class ConvertOptions(C.Structure):
+    _fields_ = [("cells", C.c_int * 3), ("delta_e", C.c_float), ("stress_in_kbar", C.c_int)]
+
+
 def build(jobs: int = 8) -> None:
     """Compile libgpuga.so and the gpuga CLI for gfx950 (hipcc; no GPU needed)."""
     subprocess.run(["make", "-s", "-C", _HERE, f"-j{jobs}"], check=True)
@@ -99,6 +103,10 @@ def lib():
             "gpuga_server_job": (C.c_int, [vp, C.c_int, ip, ip, fp, fp]),
             "gpuga_server_latencies": (C.c_int, [vp, dp, ip]),
             "gpuga_server_admissions": (C.c_int, [vp, dp, ip]),
+            "gpuga_default_convert_options": (None, [P(ConvertOptions)]),
+            "gpuga_construct_train_in": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p,
+                                                   P(ConvertOptions)]),
+            "gpuga_expand_train_in": (C.c_int, [C.c_char_p, C.c_char_p, ip]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -331,6 +339,30 @@ def read_ga_in(input_dir):
                                   C.byref(rate), C.byref(seed)))
     return {"maximum_generation": G.value, "population_size": P.value,
             "parent_number": par.value, "mutation_rate": rate.value, "seed": seed.value}
+
+
+def construct_train_in(list_path, out_path, cells=(4, 4, 3), delta_e=1.31684,
+                       stress_in_kbar=True):
+    """tools/construct_train.in/construct_train_in.cpp: DFT data files -> train.in.
+
+    list_path holds what that tool reads on standard input (Nc_force
+    Nc_energy, then the file names, relative to list_path's directory)."""
+    o = ConvertOptions()
+    lib().gpuga_default_convert_options(C.byref(o))
+    o.cells[:] = [int(c) for c in cells]
+    o.delta_e = float(delta_e)
+    o.stress_in_kbar = int(bool(stress_in_kbar))
+    with open(list_path, "rb") as f:
+        text = f.read()
+    d = os.path.dirname(os.path.abspath(list_path)).encode()
+    _check(lib().gpuga_construct_train_in(text, d, str(out_path).encode(), C.byref(o)))
+
+
+def expand_train_in(in_path, out_path, cells=(2, 2, 2)):
+    """tools/construct_train.in/expand222.cpp: every cell replicated."""
+    c = np.array(cells, np.int32)
+    _check(lib().gpuga_expand_train_in(str(in_path).encode(), str(out_path).encode(),
+                                       _ptr(c, C.c_int)))
 
 
 def run(input_dir, **opts):

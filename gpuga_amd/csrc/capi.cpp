@@ -6,6 +6,7 @@
 #include <cstring>
 #include <string>
 
+#include "convert.h"
 #include "fitness.h"
 #include "ga.h"
 #include "serve.h"
@@ -415,6 +416,60 @@ int gpuga_fitness_run_ga(gpuga_fitness* h, gpuga_ga* g, int first_generation, in
     need(g, "ga");
     if (generations < 0) fail("generations must be >= 0");
     h->f->run_ga(*g->g, first_generation, generations, best, elite);
+  });
+}
+
+void gpuga_default_convert_options(gpuga_convert_options* o)
+{
+  if (!o) return;
+  const ConvertOptions d;
+  for (int k = 0; k < 3; ++k) o->cells[k] = d.cells[k];
+  o->delta_e = d.delta_e;
+  o->stress_in_kbar = d.stress_in_kbar ? 1 : 0;
+}
+
+namespace {
+void write_text(const char* path, const std::string& s)
+{
+  FILE* f = std::fopen(path, "wb");
+  if (!f) fail(strfmt("Failed to open %s!", path));
+  const size_t n = std::fwrite(s.data(), 1, s.size(), f);
+  if (std::fclose(f) != 0 || n != s.size()) fail(strfmt("cannot write %s", path));
+}
+} // namespace
+
+int gpuga_construct_train_in(const char* list_text, const char* dir, const char* out_path,
+                             const gpuga_convert_options* opt)
+{
+  return guarded([&] {
+    need(list_text, "list_text");
+    need(out_path, "out_path");
+    ConvertOptions o;
+    if (opt) {
+      for (int k = 0; k < 3; ++k) o.cells[k] = opt->cells[k];
+      o.delta_e = opt->delta_e;
+      o.stress_in_kbar = opt->stress_in_kbar != 0;
+    }
+    write_text(out_path, construct_train_in(list_text, dir ? dir : "", o));
+  });
+}
+
+int gpuga_expand_train_in(const char* in_path, const char* out_path, const int* cells3)
+{
+  return guarded([&] {
+    need(in_path, "in_path");
+    need(out_path, "out_path");
+    const int two[3] = {2, 2, 2};
+    std::string text;
+    {
+      FILE* f = std::fopen(in_path, "rb");
+      if (!f) fail(strfmt("Failed to open %s!", in_path));
+      char buf[1 << 16];
+      size_t n;
+      while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, n);
+      std::fclose(f);
+    }
+    write_text(out_path, expand_train_in(text, cells3 ? cells3 : two));
   });
 }
 

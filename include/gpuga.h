@@ -102,6 +102,25 @@ int gpuga_fitness_neighbors(const gpuga_fitness* h, int* NN, int* start, int* NL
  * once with NULL arrays for the sizes, then with NN[N], start[N], NL[bonds]. */
 int gpuga_inspect_neighbors(const char* input_dir, int* sizes4, int* NN, int* start, int* NL);
 
+/* ----------------------------------------------------------- training-set tools
+ * The reference's offline train.in tools, host only, with their compile-time
+ * constants as options and byte-identical output.  list_text is what the
+ * converter reads on standard input (Nc_force Nc_energy, then the DFT file
+ * names, relative to dir); the result is written to out_path. */
+typedef struct gpuga_convert_options {
+  int cells[3];       /* energy/virial cell replicated NA x NB x NC (default 4 4 3) */
+  float delta_e;      /* energy shift per atom in eV (default 1.31684) */
+  int stress_in_kbar; /* 1: stress in kbar (VASP) converted to a virial in eV */
+} gpuga_convert_options;
+void gpuga_default_convert_options(gpuga_convert_options* opt);
+/* tools/construct_train.in/construct_train_in.cpp:62-222 (main, get_Nc,
+ * get_files, get_Na, get_force, get_energy_and_virial) */
+int gpuga_construct_train_in(const char* list_text, const char* dir, const char* out_path,
+                             const gpuga_convert_options* opt);
+/* tools/construct_train.in/expand222.cpp:41-151: every configuration of
+ * in_path replicated cells[0] x cells[1] x cells[2] (NULL: 2 x 2 x 2). */
+int gpuga_expand_train_in(const char* in_path, const char* out_path, const int* cells3);
+
 /* Work-item size chosen by timing candidate sizes for population P. */
 int gpuga_fitness_autotune(gpuga_fitness* h, int P, int* chunk_cost);
 

@@ -6,8 +6,11 @@ Run once in a container that has the reference checkout mounted:
 
 Only data is copied: inputs (potential.in, ga.in, train.in gzip-compressed),
 the reference's own outputs (energy/force/virial.out) and a subsample of each
-ga.out (every 50th generation plus the last two lines).  No source file of
-the reference is read or copied.
+ga.out (every 50th generation plus the last two lines).  For the two
+wurtzite cases also the DFT data their train.in was converted from (the
+converter's input.txt and the data files it names, bundled into
+dft_inputs.txt.gz: a "#list" line, then input.txt, then a "#file <name>"
+line before each file).  No source file of the reference is read or copied.
 
 Cases (reference path -> fixture directory):
   tests/si_diamond            -> si_diamond   minimal Tersoff, one species
@@ -54,7 +57,19 @@ def main(ref):
                 text = "\n".join(keep) + "\n"
             with open(os.path.join(d, name_out), "w") as fo:
                 fo.write(text)
+        if src.startswith("develop/wurzite/"):
+            bundle_dft(s, d)
         print("wrote", d)
+
+
+def bundle_dft(s, d):
+    listing = open(os.path.join(s, "input.txt")).read().replace("\r\n", "\n")
+    parts = ["#list\n", listing.rstrip("\n") + "\n"]
+    for name in listing.split()[2:]:
+        text = open(os.path.join(s, name)).read().replace("\r\n", "\n")
+        parts += ["#file %s\n" % name, text.rstrip("\n") + "\n"]
+    with gzip.GzipFile(os.path.join(d, "dft_inputs.txt.gz"), "wb", 9, mtime=0) as fo:
+        fo.write("".join(parts).encode())
 
 
 if __name__ == "__main__":
