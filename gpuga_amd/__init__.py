@@ -103,6 +103,8 @@ def lib():
             "gpuga_server_job": (C.c_int, [vp, C.c_int, ip, ip, fp, fp]),
             "gpuga_server_latencies": (C.c_int, [vp, dp, ip]),
             "gpuga_server_admissions": (C.c_int, [vp, dp, ip]),
+            "gpuga_fitness_geometry": (C.c_int, [vp, ip, fp, fp, fp]),
+            "gpuga_inspect_geometry": (C.c_int, [C.c_char_p, ip, fp, fp, fp]),
             "gpuga_default_convert_options": (None, [P(ConvertOptions)]),
             "gpuga_construct_train_in": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p,
                                                    P(ConvertOptions)]),
@@ -224,6 +226,10 @@ class Fitness:
                                              _ptr(NL, C.c_int)))
         return NN, start, NL[: self.bonds]
 
+    def geometry(self):
+        """The geometry tables built on the device: (bond, tri, rec), float4 rows."""
+        return _tables(lambda *a: lib().gpuga_fitness_geometry(self._h, *a))
+
     def autotune(self, P):
         c = C.c_int()
         _check(lib().gpuga_fitness_autotune(self._h, int(P), C.byref(c)))
@@ -329,6 +335,20 @@ def inspect_neighbors(input_dir):
     _check(lib().gpuga_inspect_neighbors(d, _ptr(sizes, C.c_int), _ptr(NN, C.c_int),
                                          _ptr(start, C.c_int), _ptr(NL, C.c_int)))
     return NN, start, NL[:B]
+
+
+def _tables(call):
+    sizes = np.zeros(3, np.int32)
+    _check(call(_ptr(sizes, C.c_int), None, None, None))
+    t = [np.zeros((max(int(n), 1), 4), np.float32) for n in sizes]
+    _check(call(_ptr(sizes, C.c_int), *[_ptr(a) for a in t]))
+    return tuple(a[:int(n)] for a, n in zip(t, sizes))
+
+
+def inspect_geometry(input_dir):
+    """Host-built geometry tables of a job directory: (bond, tri, rec), float4 rows."""
+    d = str(input_dir).encode()
+    return _tables(lambda *a: lib().gpuga_inspect_geometry(d, *a))
 
 
 def read_ga_in(input_dir):

@@ -281,6 +281,57 @@ int gpuga_inspect_neighbors(const char* dir, int* sizes, int* NN, int* start, in
   });
 }
 
+namespace {
+void put_tables(int* sizes, float* bond, float* tri, float* rec, const std::vector<float4>& b,
+                const std::vector<float4>& t, const std::vector<float4>& r)
+{
+  sizes[0] = (int)b.size();
+  sizes[1] = (int)t.size();
+  sizes[2] = (int)r.size();
+  if (bond) std::memcpy(bond, b.data(), sizeof(float4) * b.size());
+  if (tri) std::memcpy(tri, t.data(), sizeof(float4) * t.size());
+  if (rec) std::memcpy(rec, r.data(), sizeof(float4) * r.size());
+}
+} // namespace
+
+int gpuga_fitness_geometry(const gpuga_fitness* h, int* sizes, float* bond, float* tri, float* rec)
+{
+  return guarded([&] {
+    need(h, "handle");
+    need(sizes, "sizes");
+    std::vector<float4> b, t, r;
+    h->f->device_geometry(b, t, r);
+    put_tables(sizes, bond, tri, rec, b, t, r);
+  });
+}
+
+int gpuga_inspect_geometry(const char* dir, int* sizes, float* bond, float* tri, float* rec)
+{
+  return guarded([&] {
+    need(dir, "input_dir");
+    need(sizes, "sizes");
+    const PotentialSpec s = read_potential(dir);
+    const TrainSet ts = read_train(dir);
+    const NeighborList nl = build_neighbors(ts, s.cutoff);
+    const bool tersoff = s.type == POT_TERSOFF1 || s.type == POT_TERSOFF2;
+    Geometry g = build_geometry(ts, nl, tersoff);
+    std::vector<float4> b(std::max(nl.total, 1), make_float4(0, 0, 0, 1)), t(1, make_float4(0, 0, 0, 0)), r;
+    std::memcpy(b.data(), g.bond.data(), sizeof(float4) * g.bond.size());
+    if (tersoff) {
+      if (g.tri_total > 0) {
+        t.resize(g.tri.size());
+        std::memcpy(t.data(), g.tri.data(), sizeof(float4) * g.tri.size());
+      }
+      if (const int M = record_slots(nl.max_nn)) {
+        const std::vector<Vec4> rv = build_records(nl, g, bond_types(ts, nl, true), M);
+        r.resize(rv.size());
+        std::memcpy(r.data(), rv.data(), sizeof(float4) * rv.size());
+      }
+    }
+    put_tables(sizes, bond, tri, rec, b, t, r);
+  });
+}
+
 int gpuga_fitness_autotune(gpuga_fitness* h, int P, int* chunk_cost)
 {
   return guarded([&] {

@@ -164,6 +164,18 @@ Fitness::~Fitness()
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+void Fitness::device_geometry(std::vector<float4>& bond, std::vector<float4>& tri,
+                              std::vector<float4>& rec) const
+{
+  bond.resize(ds_.bond.size());
+  tri.resize(ds_.tri.size());
+  rec.resize(ds_.rec.size());
+  ds_.bond.download(bond.data(), bond.size(), stream_);
+  ds_.tri.download(tri.data(), tri.size(), stream_);
+  ds_.rec.download(rec.data(), rec.size(), stream_);
+  GPUGA_HIP(hipStreamSynchronize(stream_));
+}
+
 void Fitness::setup()
 {
   int ndev = 0;
@@ -186,21 +198,9 @@ void Fitness::setup()
     for (int s = 0; s < nl_.NN[i]; ++s)
       if (nl_.NL[nl_.rev[nl_.start[i] + s]] != i)
         fail("neighbour list is not symmetric (reverse bond missing)");
-  geo_ = build_geometry(ts_, nl_, tersoff);
-  dmin_ = 1e30f;
-  dmax_ = 0.0f;
-  for (const Vec4& b : geo_.bond) {
-    dmin_ = std::min(dmin_, b.w);
-    dmax_ = std::max(dmax_, b.w);
-  }
-  if (geo_.bond.empty()) dmin_ = dmax_ = 1.0f;
-
-  std::vector<int> ptype(nl_.total);
-  for (int i = 0; i < ts_.N; ++i)
-    for (int s = 0; s < nl_.NN[i]; ++s) {
-      const int b = nl_.start[i] + s, j = nl_.NL[b];
-      ptype[b] = tersoff ? (ts_.type[i] != ts_.type[j] ? 1 : 0) : ts_.type[i] + ts_.type[j];
-    }
+  // pair types on the host (the list is there); the geometry tables and the
+  // atom records on the device, from the device copy of the list
+  const std::vector<int> ptype = bond_types(ts_, nl_, tersoff);
 
   hipStream_t st = stream_;
   ds_.Na.upload(ts_.Na, st);
@@ -211,43 +211,52 @@ void Fitness::setup()
   ds_.nl.upload(nl_.NL.empty() ? std::vector<int>{0} : nl_.NL, st);
   ds_.rev.upload(nl_.rev.empty() ? std::vector<int>{0} : nl_.rev, st);
   ds_.ptype.upload(ptype.empty() ? std::vector<int>{0} : ptype, st);
-  static_assert(sizeof(Vec4) == sizeof(float4), "layout");
-  if (geo_.bond.empty()) geo_.bond.push_back(Vec4{0, 0, 0, 1});
-  ds_.bond.upload((const float4*)geo_.bond.data(), geo_.bond.size(), st);
+  DeviceNeighbors dn;
+  dn.NN = ds_.nn.get();
+  dn.start = ds_.bond0.get();
+  dn.NL = ds_.nl.get();
+  dn.rev = ds_.rev.get();
+  dn.total = nl_.total;
+  if (nl_.total == 0) ds_.bond.upload(std::vector<float4>{make_float4(0, 0, 0, 1)}, st);
+  else ds_.bond.resize(nl_.total);
+  std::vector<int> tri0;
   if (tersoff) {
-    ds_.tri0.upload(geo_.tri_start, st);
-    if (geo_.tri.empty()) geo_.tri.push_back(Vec4{0, 0, 0, 0});
-    ds_.tri.upload((const float4*)geo_.tri.data(), geo_.tri.size(), st);
-    if (nl_.max_nn <= 16) {
-      // fixed-stride atom records for the population kernel (see DevSet::rec)
-      const int M = nl_.max_nn <= 4 ? 4 : (nl_.max_nn <= 8 ? 8 : 16);
+    long long tri_total = 0;
+    tri0 = triplet_offsets(nl_, &tri_total);
+    ds_.tri0.upload(tri0, st);
+    if (tri_total == 0) ds_.tri.upload(std::vector<float4>(1), st);
+    else ds_.tri.resize((size_t)tri_total);
+    dn.tri0 = ds_.tri0.get();
+    build_geometry_device(ts_, dn, ds_.bond.get(), ds_.tri.get(), st);
+    if (const int M = record_slots(nl_.max_nn)) {
       const size_t RS = 1 + M + (size_t)M * M + M / 2;
-      std::vector<Vec4> rec((size_t)ts_.N * RS, Vec4{0, 0, 0, 0});
-      for (int i = 0; i < ts_.N; ++i) {
-        const int nn = nl_.NN[i], b0 = nl_.start[i];
-        Vec4* R = &rec[(size_t)i * RS];
-        int hdr[4] = {nn, 0, b0, 0};
-        for (int u = 0; u < nn; ++u) {
-          hdr[1] |= (ptype[b0 + u] & 1) << u;
-          R[1 + u] = geo_.bond[b0 + u];
-          for (int w = 0; w < nn; ++w)
-            if (w != u) R[1 + M + u * M + w] = geo_.tri[geo_.tri_start[i] + u * nn + w];
-        }
-        std::memcpy(&R[0], hdr, sizeof hdr);
-        int* rv = reinterpret_cast<int*>(&R[1 + M + M * M]);
-        for (int u = 0; u < nn; ++u) rv[u] = nl_.rev[b0 + u];
-        float* inv = reinterpret_cast<float*>(&R[1 + M + M * M + M / 4]);
-        for (int u = 0; u < nn; ++u) inv[u] = (float)(1.0 / (double)geo_.bond[b0 + u].w);
-      }
-      ds_.rec.upload((const float4*)rec.data(), rec.size(), st);
+      ds_.rec.resize((size_t)ts_.N * RS);
+      build_records_device(ts_.N, M, dn, ds_.ptype.get(), ds_.bond.get(), ds_.tri.get(),
+                           ds_.rec.get(), st);
       ds_.view.rec_nmax = M;
       ds_.view.bond_force =
         ts_.N_force > 0 ? nl_.start[ts_.N_force - 1] + nl_.NN[ts_.N_force - 1] : 0;
     }
+    dmin_ = dmax_ = 1.0f; // (the NN2B tabulation range; unused here)
   } else {
     ds_.tri0.upload(std::vector<int>(ts_.N, 0), st);
     ds_.tri.upload(std::vector<float4>(1), st);
+    build_geometry_device(ts_, dn, ds_.bond.get(), nullptr, st);
+    // the pair tables below are built on the host from the bond geometry
+    geo_.bond.resize(ds_.bond.size());
+    static_assert(sizeof(Vec4) == sizeof(float4), "layout");
+    ds_.bond.download(reinterpret_cast<float4*>(geo_.bond.data()), geo_.bond.size(), st);
+    GPUGA_HIP(hipStreamSynchronize(st));
+    dmin_ = 1e30f;
+    dmax_ = 0.0f;
+    for (int b = 0; b < nl_.total; ++b) {
+      dmin_ = std::min(dmin_, geo_.bond[b].w);
+      dmax_ = std::max(dmax_, geo_.bond[b].w);
+    }
+    if (nl_.total == 0) dmin_ = dmax_ = 1.0f;
   }
+  // the uploads above read host vectors of this scope
+  GPUGA_HIP(hipStreamSynchronize(st));
   if (spec_.type == POT_RI) {
     // per-bond factors of src/ri.cu:80-101 that do not depend on parameters
     std::vector<float4> tab(std::max(nl_.total, 1));

@@ -123,7 +123,9 @@ public:
   const PotentialSpec& spec() const { return spec_; }
   const TrainSet& trainset() const { return ts_; }
   const NeighborList& neighbors() const { return nl_; }
-  const Geometry& geometry() const { return geo_; }
+  // The device geometry tables (bond, triplet, atom records), copied back.
+  void device_geometry(std::vector<float4>& bond, std::vector<float4>& tri,
+                       std::vector<float4>& rec) const;
   hipStream_t stream() const { return stream_; }
   size_t device_bytes() const;
   size_t plan_bytes(int P) const;
@@ -152,7 +154,7 @@ private:
   PotentialSpec spec_;
   TrainSet ts_;
   NeighborList nl_;
-  Geometry geo_;
+  Geometry geo_; // host bond geometry of the pair potentials' set-up
   DeviceSet ds_;
   std::unique_ptr<Potential> potential_;
   std::map<int, std::unique_ptr<PopPlan>> plans_;

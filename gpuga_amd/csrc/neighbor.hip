@@ -8,6 +8,11 @@
 // the atom's configuration, ascending), write them at the offsets of the
 // exclusive prefix sum of the counts (computed on the host, which needs the
 // list anyway), then find each bond's reverse bond.
+//
+// The parameter-independent geometry tables follow on the device as well,
+// bit-identical to build_geometry / build_records (trainset.cpp): the same
+// FP64 expressions without contraction (IEEE double add, multiply, divide and
+// square root are correctly rounded on both sides), rounded to float.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -95,6 +100,91 @@ __global__ void __launch_bounds__(256) nl_reverse(int N, const int* NN, const in
   }
 }
 
+// Minimum image of bond k = (i, j) in FP64 on the float box, as
+// build_geometry: rb[k] = (x, y, z, |r|), bond[k] = the same rounded to float.
+__global__ void __launch_bounds__(256) geo_bonds(NLView v, const int* NN, const int* start,
+                                                 const int* NL, double4* rb, float4* bond)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.N) return;
+  const float* h = v.h + 18 * (long)v.cfg[i];
+  const long N = v.N;
+  const int n = NN[i], b0 = start[i];
+  for (int s = 0; s < n; ++s) {
+    const int k = b0 + s, j = NL[k];
+    double x = (double)v.r[j] - (double)v.r[i];
+    double y = (double)v.r[j + N] - (double)v.r[i + N];
+    double z = (double)v.r[j + 2 * N] - (double)v.r[i + 2 * N];
+    double sx = (double)h[9] * x + (double)h[10] * y + (double)h[11] * z;
+    double sy = (double)h[12] * x + (double)h[13] * y + (double)h[14] * z;
+    double sz = (double)h[15] * x + (double)h[16] * y + (double)h[17] * z;
+    sx -= __builtin_rint(sx);
+    sy -= __builtin_rint(sy);
+    sz -= __builtin_rint(sz);
+    x = (double)h[0] * sx + (double)h[1] * sy + (double)h[2] * sz;
+    y = (double)h[3] * sx + (double)h[4] * sy + (double)h[5] * sz;
+    z = (double)h[6] * sx + (double)h[7] * sy + (double)h[8] * sz;
+    const double d = __builtin_sqrt(x * x + y * y + z * z);
+    rb[k] = make_double4(x, y, z, d);
+    bond[k] = make_float4((float)x, (float)y, (float)z, (float)d);
+  }
+}
+
+// cos(theta_jik) and its gradient with respect to r_ij for every ordered
+// pair (s, t) of atom i's bonds, zero where s == t
+__global__ void __launch_bounds__(256) geo_triplets(int N, const int* NN, const int* start,
+                                                    const int* tri0, const double4* rb, float4* tri)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int nn = NN[i], b0 = start[i];
+  float4* out = tri + tri0[i];
+  for (int s = 0; s < nn; ++s) {
+    const double4 a = rb[b0 + s];
+    for (int t = 0; t < nn; ++t) {
+      if (s == t) {
+        out[s * nn + t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        continue;
+      }
+      const double4 b = rb[b0 + t];
+      const double inv = 1.0 / (a.w * b.w);
+      const double cs = (a.x * b.x + a.y * b.y + a.z * b.z) * inv;
+      const double cd = cs / (a.w * a.w);
+      out[s * nn + t] = make_float4((float)cs, (float)(b.x * inv - a.x * cd),
+                                    (float)(b.y * inv - a.y * cd), (float)(b.z * inv - a.z * cd));
+    }
+  }
+}
+
+// Atom records of the Tersoff population kernel, the layout of
+// build_records: header (nn, species-pair mask, first bond, 0), M bonds,
+// M x M triplets, M reverse bonds, M inverse distances, zero-padded.
+__global__ void __launch_bounds__(256) geo_records(int N, int M, const int* NN, const int* start,
+                                                   const int* rev, const int* ptype, const int* tri0,
+                                                   const float4* bond, const float4* tri, float4* rec)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int nn = NN[i], b0 = start[i];
+  const long RS = 1 + M + (long)M * M + M / 2;
+  float4* R = rec + (long)i * RS;
+  int mask = 0;
+  for (int u = 0; u < nn; ++u) mask |= (ptype[b0 + u] & 1) << u;
+  R[0] = make_float4(__int_as_float(nn), __int_as_float(mask), __int_as_float(b0), 0.0f);
+  const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int u = 0; u < M; ++u) R[1 + u] = u < nn ? bond[b0 + u] : zero;
+  const float4* T = tri + tri0[i];
+  for (int u = 0; u < M; ++u)
+    for (int w = 0; w < M; ++w)
+      R[1 + M + u * M + w] = u < nn && w < nn && u != w ? T[u * nn + w] : zero;
+  int* rv = reinterpret_cast<int*>(R + 1 + M + M * M);
+  float* inv = reinterpret_cast<float*>(R + 1 + M + M * M + M / 4);
+  for (int u = 0; u < M; ++u) {
+    rv[u] = u < nn ? rev[b0 + u] : 0;
+    inv[u] = u < nn ? (float)(1.0 / (double)bond[b0 + u].w) : 0.0f;
+  }
+}
+
 #pragma clang fp contract(on)
 
 // Device copies of the positions, boxes and configuration index.
@@ -142,6 +232,32 @@ std::vector<int> count_neighbors_device(const TrainSet& ts, float cutoff, hipStr
   in.NN.download(nn.data(), nn.size(), st);
   GPUGA_HIP(hipStreamSynchronize(st));
   return nn;
+}
+
+void build_geometry_device(const TrainSet& ts, const DeviceNeighbors& d, float4* bond,
+                           float4* tri, hipStream_t st)
+{
+  if (d.total == 0 || ts.N == 0) return;
+  NLInput in(ts, 0.0f, st);
+  DeviceBuffer<double4> rb(d.total);
+  geo_bonds<<<in.blocks(), 256, 0, st>>>(in.view, d.NN, d.start, d.NL, rb.get(), bond);
+  GPUGA_HIP(hipGetLastError());
+  if (tri) {
+    geo_triplets<<<in.blocks(), 256, 0, st>>>(ts.N, d.NN, d.start, d.tri0, rb.get(), tri);
+    GPUGA_HIP(hipGetLastError());
+  }
+  // rb and the inputs are freed on return
+  GPUGA_HIP(hipStreamSynchronize(st));
+}
+
+void build_records_device(int N, int M, const DeviceNeighbors& d, const int* ptype,
+                          const float4* bond, const float4* tri, float4* rec, hipStream_t st)
+{
+  if (N == 0) return;
+  if (M != 4 && M != 8 && M != 16) fail("atom records hold 4, 8 or 16 bonds");
+  geo_records<<<(N + 255) / 256, 256, 0, st>>>(N, M, d.NN, d.start, d.rev, ptype, d.tri0, bond,
+                                               tri, rec);
+  GPUGA_HIP(hipGetLastError());
 }
 
 NeighborList build_neighbors_device(const TrainSet& ts, float cutoff, hipStream_t st)

@@ -468,13 +468,8 @@ Geometry build_geometry(const TrainSet& ts, const NeighborList& nl, bool triplet
       }
   });
   if (!triplets) return g;
-  g.tri_start.resize(N);
   long long total = 0;
-  for (int i = 0; i < N; ++i) {
-    g.tri_start[i] = (int)total;
-    total += (long long)nl.NN[i] * nl.NN[i];
-  }
-  if (total > 0x7fffffffLL) fail("triplet table too large");
+  g.tri_start = triplet_offsets(nl, &total);
   g.tri_total = total;
   g.tri.assign(total, Vec4{0, 0, 0, 0});
   parallel_for(N, [&](int i) {
@@ -494,6 +489,60 @@ Geometry build_geometry(const TrainSet& ts, const NeighborList& nl, bool triplet
       }
   });
   return g;
+}
+
+std::vector<int> triplet_offsets(const NeighborList& nl, long long* total)
+{
+  std::vector<int> off(nl.NN.size());
+  long long t = 0;
+  for (size_t i = 0; i < nl.NN.size(); ++i) {
+    off[i] = (int)t;
+    t += (long long)nl.NN[i] * nl.NN[i];
+    if (t > 0x7fffffffLL) fail("triplet table too large");
+  }
+  *total = t;
+  return off;
+}
+
+std::vector<int> bond_types(const TrainSet& ts, const NeighborList& nl, bool three_body)
+{
+  std::vector<int> p(nl.total);
+  for (int i = 0; i < ts.N; ++i)
+    for (int s = 0; s < nl.NN[i]; ++s) {
+      const int b = nl.start[i] + s, j = nl.NL[b];
+      p[b] = three_body ? (ts.type[i] != ts.type[j] ? 1 : 0) : ts.type[i] + ts.type[j];
+    }
+  return p;
+}
+
+int record_slots(int max_nn)
+{
+  return max_nn <= 4 ? 4 : max_nn <= 8 ? 8 : max_nn <= 16 ? 16 : 0;
+}
+
+std::vector<Vec4> build_records(const NeighborList& nl, const Geometry& g,
+                                const std::vector<int>& ptype, int M)
+{
+  const int N = (int)nl.NN.size();
+  const size_t RS = 1 + M + (size_t)M * M + M / 2;
+  std::vector<Vec4> rec((size_t)N * RS, Vec4{0, 0, 0, 0});
+  parallel_for(N, [&](int i) {
+    const int nn = nl.NN[i], b0 = nl.start[i];
+    Vec4* R = &rec[(size_t)i * RS];
+    int hdr[4] = {nn, 0, b0, 0};
+    for (int u = 0; u < nn; ++u) {
+      hdr[1] |= (ptype[b0 + u] & 1) << u;
+      R[1 + u] = g.bond[b0 + u];
+      for (int w = 0; w < nn; ++w)
+        if (w != u) R[1 + M + u * M + w] = g.tri[g.tri_start[i] + u * nn + w];
+    }
+    std::memcpy(&R[0], hdr, sizeof hdr);
+    int* rv = reinterpret_cast<int*>(&R[1 + M + M * M]);
+    for (int u = 0; u < nn; ++u) rv[u] = nl.rev[b0 + u];
+    float* inv = reinterpret_cast<float*>(&R[1 + M + M * M + M / 4]);
+    for (int u = 0; u < nn; ++u) inv[u] = (float)(1.0 / (double)g.bond[b0 + u].w);
+  });
+  return rec;
 }
 
 } // namespace gpuga

@@ -79,6 +79,30 @@ NeighborList build_neighbors_device(const TrainSet& ts, float cutoff, hipStream_
 // Its per-atom neighbour counts alone.
 std::vector<int> count_neighbors_device(const TrainSet& ts, float cutoff, hipStream_t st);
 Geometry build_geometry(const TrainSet& ts, const NeighborList& nl, bool triplets);
+// Offsets of each atom's NN x NN triplet block and their total.
+std::vector<int> triplet_offsets(const NeighborList& nl, long long* total);
+// Per-bond pair type: three-body potentials 1 when the species differ (else
+// 0), pair potentials t_i + t_j.
+std::vector<int> bond_types(const TrainSet& ts, const NeighborList& nl, bool three_body);
+// Bond slots M (4, 8 or 16) of the fixed-stride atom records of the Tersoff
+// population kernel, 0 when an atom has more than 16 neighbours; a record
+// holds 1 + M + M*M + M/2 float4: header ints (nn, species-pair bit mask,
+// first bond, 0), M bonds, M x M triplets (s, t), M reverse-bond ints and M
+// inverse distances, zero-padded.
+int record_slots(int max_nn);
+std::vector<Vec4> build_records(const NeighborList& nl, const Geometry& g,
+                                const std::vector<int>& ptype, int M);
+
+// The device forms (neighbor.hip), bit-identical to the host ones above.
+struct DeviceNeighbors {
+  const int *NN = nullptr, *start = nullptr, *NL = nullptr, *rev = nullptr, *tri0 = nullptr;
+  int total = 0;
+};
+// bond[total]; tri[triplet total] when tri != nullptr (needs d.tri0)
+void build_geometry_device(const TrainSet& ts, const DeviceNeighbors& d, float4* bond,
+                           float4* tri, hipStream_t st);
+void build_records_device(int N, int M, const DeviceNeighbors& d, const int* ptype,
+                          const float4* bond, const float4* tri, float4* rec, hipStream_t st);
 
 // Index of the configuration that holds atom i.
 int config_of(const TrainSet& ts, int i);

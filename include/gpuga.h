@@ -121,6 +121,17 @@ int gpuga_construct_train_in(const char* list_text, const char* dir, const char*
  * in_path replicated cells[0] x cells[1] x cells[2] (NULL: 2 x 2 x 2). */
 int gpuga_expand_train_in(const char* in_path, const char* out_path, const int* cells3);
 
+/* Inspection of the parameter-independent geometry tables the population
+ * kernels read (the reference recomputes this geometry in every kernel,
+ * src/minimal_tersoff.cu:150-153,162-167, src/mic.cuh:20-32).  Arrays of
+ * float4: bond[sizes3[0]] (r_ij, |r_ij|), tri[sizes3[1]] (cos theta_jik and
+ * its gradient), rec[sizes3[2]] (Tersoff atom records).  Call once with NULL
+ * arrays for the sizes.  gpuga_fitness_geometry copies the tables built on
+ * the device; gpuga_inspect_geometry builds them on the host (no HIP device
+ * used) from potential.in and train.in. */
+int gpuga_fitness_geometry(const gpuga_fitness* h, int* sizes3, float* bond, float* tri, float* rec);
+int gpuga_inspect_geometry(const char* input_dir, int* sizes3, float* bond, float* tri, float* rec);
+
 /* Work-item size chosen by timing candidate sizes for population P. */
 int gpuga_fitness_autotune(gpuga_fitness* h, int P, int* chunk_cost);
 
