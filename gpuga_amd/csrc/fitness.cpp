@@ -257,43 +257,57 @@ void Fitness::setup()
   }
   // the uploads above read host vectors of this scope
   GPUGA_HIP(hipStreamSynchronize(st));
+  // the pairs a pair potential evaluates for atom i: all its bonds in force
+  // configurations (forces need both directions), j > i elsewhere (energy
+  // and virial need each pair once); pair_offsets()[i] = the first of atom
+  // i's pairs, [N] = their number
+  auto evaluates = [&](int i, int b) { return i < ts_.N_force || nl_.NL[b] > i; };
+  auto pair_offsets = [&] {
+    std::vector<int> off(ts_.N + 1, 0);
+    parallel_for(ts_.N, [&](int i) {
+      int c = 0;
+      for (int b = nl_.start[i]; b < nl_.start[i] + nl_.NN[i]; ++b) c += evaluates(i, b);
+      off[i + 1] = c;
+    });
+    for (int i = 0; i < ts_.N; ++i) off[i + 1] += off[i];
+    return off;
+  };
   if (spec_.type == POT_RI) {
     // per-bond factors of src/ri.cu:80-101 that do not depend on parameters
     std::vector<float4> tab(std::max(nl_.total, 1));
-    for (int b = 0; b < nl_.total; ++b) {
-      const double d = geo_.bond[b].w;
-      const double dinv = 1.0 / d;
-      const double e = std::erfc(0.2 * d) * dinv;
-      const double g = e * dinv + 0.225675833419103 * std::exp(-0.04 * d * d) * dinv;
-      tab[b] = make_float4((float)std::pow(dinv, 6), (float)e, (float)g, (float)dinv);
-    }
+    parallel_for(ts_.N, [&](int i) {
+      for (int b = nl_.start[i]; b < nl_.start[i] + nl_.NN[i]; ++b) {
+        const double d = geo_.bond[b].w;
+        const double dinv = 1.0 / d;
+        const double e = std::erfc(0.2 * d) * dinv;
+        const double g = e * dinv + 0.225675833419103 * std::exp(-0.04 * d * d) * dinv;
+        tab[b] = make_float4((float)std::pow(dinv, 6), (float)e, (float)g, (float)dinv);
+      }
+    });
     ds_.ri_tab.upload(tab, st);
     // population form: per atom the pairs it evaluates (all bonds for force
     // configurations, those with j > i for the others), grouped by pair type
     // t_i + t_j, each as unit vector and d, then the factors plus 6 d^-7
-    std::vector<float4> pt;
-    pt.reserve(2 * (size_t)nl_.total);
+    const std::vector<int> p0 = pair_offsets();
+    std::vector<float4> pt(2 * (size_t)std::max(p0[ts_.N], 1));
     std::vector<int4> hdr(std::max(ts_.N, 1));
-    for (int i = 0; i < ts_.N; ++i) {
+    parallel_for(ts_.N, [&](int i) {
       const int nn = nl_.NN[i], b0 = nl_.start[i];
-      const bool all = i < ts_.N_force;
       int cnt[3] = {0, 0, 0};
-      const int start = (int)(pt.size() / 2);
+      float4* out = &pt[2 * (size_t)p0[i]];
       for (int t = 0; t < 3; ++t)
         for (int u = 0; u < nn; ++u) {
           const int b = b0 + u;
-          if (ptype[b] != t || (!all && nl_.NL[b] < i)) continue;
+          if (ptype[b] != t || !evaluates(i, b)) continue;
           const Vec4 r = geo_.bond[b];
           const double dinv = 1.0 / (double)r.w;
           const double d6 = std::pow(dinv, 6);
-          pt.push_back(make_float4((float)(r.x * dinv), (float)(r.y * dinv),
-                                   (float)(r.z * dinv), r.w));
-          pt.push_back(make_float4((float)d6, (float)(6.0 * d6 * dinv), tab[b].y, tab[b].z));
+          *out++ = make_float4((float)(r.x * dinv), (float)(r.y * dinv), (float)(r.z * dinv), r.w);
+          *out++ = make_float4((float)d6, (float)(6.0 * d6 * dinv), tab[b].y, tab[b].z);
           ++cnt[t];
         }
-      hdr[i] = make_int4(start, cnt[0], cnt[1], cnt[2]);
-    }
-    if (pt.empty()) pt.resize(2);
+      hdr[i] = make_int4(p0[i], cnt[0], cnt[1], cnt[2]);
+    });
     ds_.pair_tab.upload(pt, st);
     ds_.pair_hdr.upload(hdr, st);
   }
@@ -305,19 +319,19 @@ void Fitness::setup()
     // grid (h00, h h10, h01, h h11), and (fc, fc', grid cell, 0) with the
     // reference's fixed taper (src/nn2b.cu:80-92)
     const NN2BGrid grid = nn2b_grid_desc();
-    std::vector<float4> pt;
-    pt.reserve(3 * (size_t)nl_.total);
+    const std::vector<int> p0 = pair_offsets();
+    std::vector<float4> pt(3 * (size_t)std::max(p0[ts_.N], 1));
     std::vector<int4> hdr(std::max(ts_.N, 1));
-    std::vector<int> order;
-    for (int i = 0; i < ts_.N; ++i) {
+    parallel_for(ts_.N, [&](int i) {
       const int nn = nl_.NN[i], b0 = nl_.start[i];
-      const bool all = i < ts_.N_force;
-      order.clear();
+      std::vector<int> order;
+      order.reserve(nn);
       for (int u = 0; u < nn; ++u)
-        if (all || nl_.NL[b0 + u] > i) order.push_back(b0 + u);
+        if (evaluates(i, b0 + u)) order.push_back(b0 + u);
       std::stable_sort(order.begin(), order.end(),
                        [&](int a, int b) { return geo_.bond[a].w < geo_.bond[b].w; });
-      hdr[i] = make_int4((int)(pt.size() / 3), (int)order.size(), 0, 0);
+      hdr[i] = make_int4(p0[i], (int)order.size(), 0, 0);
+      float4* out = &pt[3 * (size_t)p0[i]];
       for (int b : order) {
         const Vec4 r = geo_.bond[b];
         const double d = r.w, dinv = 1.0 / d;
@@ -332,16 +346,14 @@ void Fitness::setup()
           fc = 0.5 * std::cos(a) + 0.5;
           fcp = -0.5 * (double)NN2B_PI_FACTOR * std::sin(a);
         }
-        pt.push_back(make_float4((float)(r.x * dinv), (float)(r.y * dinv), (float)(r.z * dinv),
-                                 r.w));
-        pt.push_back(make_float4((float)(2 * t3 - 3 * t2 + 1), (float)(h * (t3 - 2 * t2 + t)),
-                                 (float)(-2 * t3 + 3 * t2), (float)(h * (t3 - t2))));
+        *out++ = make_float4((float)(r.x * dinv), (float)(r.y * dinv), (float)(r.z * dinv), r.w);
+        *out++ = make_float4((float)(2 * t3 - 3 * t2 + 1), (float)(h * (t3 - 2 * t2 + t)),
+                             (float)(-2 * t3 + 3 * t2), (float)(h * (t3 - t2)));
         float4 c = make_float4((float)fc, (float)fcp, 0.0f, 0.0f);
         std::memcpy(&c.z, &g, sizeof g);
-        pt.push_back(c);
+        *out++ = c;
       }
-    }
-    if (pt.empty()) pt.resize(3);
+    });
     ds_.pair_tab.upload(pt, st);
     ds_.pair_hdr.upload(hdr, st);
   }
