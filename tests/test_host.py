@@ -149,3 +149,49 @@ def test_ga_in_comments_defaults_and_seed(tmp_path):
                   "mutation_rate": pytest.approx(0.2), "seed": 99}
     og = O.lib()
     del og
+
+
+@pytest.mark.parametrize("name", ["si_diamond", "gaas", "hfo2"])
+def test_host_geometry_tables(golden, name):
+    """The host geometry tables (the reference the device build is compared
+    with bit for bit): bond vectors of reverse bonds are exact negatives
+    (the minimum image is odd in IEEE arithmetic), lengths are the rounded
+    norms, triplet cosines are symmetric bit for bit and equal the cosine of
+    the bond vectors, and the gradient is orthogonal to r_ij (d cos / d r_ij
+    . r_ij = 0); Tersoff records repeat the tables in their slots."""
+    d = golden(name)
+    bond, tri, rec = G.inspect_geometry(d)
+    NN, start, NL = G.inspect_neighbors(d)
+    N, B = len(NN), len(NL)
+    bond = bond[:B].astype(np.float64)
+    owner = np.repeat(np.arange(N), NN)
+    # reverse bonds: keys i N + j are ascending over the list (atoms in
+    # order, each list ascending), so the slot of (j, i) is a binary search
+    keys = owner.astype(np.int64) * N + NL
+    rev = np.searchsorted(keys, NL.astype(np.int64) * N + owner)
+    assert np.array_equal(NL[rev], owner)
+    assert np.array_equal(bond[rev, :3], -bond[:, :3])
+    norm = np.sqrt((bond[:, :3] ** 2).sum(1))
+    np.testing.assert_allclose(bond[:, 3], norm, rtol=2e-7)
+    if name == "hfo2":
+        assert len(rec) == 0
+        return
+    for i in range(0, N, max(1, N // 200)):
+        nn, b0 = NN[i], start[i]
+        off = int(np.sum(NN[:i].astype(np.int64) ** 2))
+        T = tri[off:off + nn * nn].reshape(nn, nn, 4)
+        u = bond[b0:b0 + nn, :3] / bond[b0:b0 + nn, 3:4]
+        for s in range(nn):
+            for t in range(nn):
+                if s == t:
+                    assert not T[s, t].any()
+                    continue
+                assert T[s, t, 0].view(np.uint32) == T[t, s, 0].view(np.uint32)
+                assert T[s, t, 0] == pytest.approx(u[s] @ u[t], abs=1e-6)
+                assert abs(T[s, t, 1:] @ bond[b0 + s, :3]) < 1e-5
+        M = 4 if NN.max() <= 4 else (8 if NN.max() <= 8 else 16)
+        RS = 1 + M + M * M + M // 2
+        R = rec[i * RS:(i + 1) * RS]
+        assert R[0].view(np.int32)[0] == nn and R[0].view(np.int32)[2] == b0
+        assert np.array_equal(R[1:1 + nn], bond[b0:b0 + nn].astype(np.float32))
+        assert np.array_equal(R[1 + M:1 + M + M * M].reshape(M, M, 4)[:nn, :nn], T)
