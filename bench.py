@@ -265,7 +265,7 @@ def main():
     value = P * N * args.steps / elapsed
     kernel_s = kms * 1e-3 / max(launches, 1)
     # Algorithmic work of the reference formulation per generation (the
-    # per-unit counts of the hot-path analysis); the population kernels skip
+    # per-unit counts above, FMA = 2 FLOP); the population kernels skip
     # part of it (precomputed geometry, unordered pairs, tabulated NN2B), so
     # for the pair potentials the fraction can exceed what the VALU executes.
     if w["kind"] == "tersoff":
