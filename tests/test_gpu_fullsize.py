@@ -66,3 +66,63 @@ def test_full_size_ga_run_equals_host_loop(headline):
     best_d, _ = fit.run_ga(dev, 0, gens)
     assert np.array_equal(np.array(best_h, np.float32), best_d)
     assert np.array_equal(host.population, dev.population)
+
+
+def _labelled(make):
+    import gpuga_amd
+    from gpuga_amd import synth
+    d = make()
+    lab = gpuga_amd.Fitness(arrays=d)
+    d = synth.label(d, lab.find_force)
+    lab.close()
+    return d
+
+
+def _oracle_case(d):
+    from oracle import oracle as O
+    pot = dict(potential_type=d["potential_type"], cutoff=d["cutoff"], weights=d["weights"],
+               pmin=d["pmin"], pmax=d["pmax"], num_neurons=d.get("num_neurons", 0))
+    return O.Case(potential=pot, arrays=d)
+
+
+def test_full_size_rigid_ion_matches_oracle():
+    """The rigid-ion fluorite workload at full size (500 x 96 atoms, about 95
+    neighbours, P = 200, 15 parameters): individuals of every lane block
+    against the oracle, tolerance 1e-5 relative."""
+    import gpuga_amd
+    from gpuga_amd import synth
+    print_case("full size rigid ion vs oracle")
+    d = _labelled(lambda: synth.fluorite_ab2(Nc=500, seed=3))
+    fit = gpuga_amd.Fitness(arrays=d)
+    try:
+        genes = np.random.default_rng(79).random((200, fit.V), dtype=np.float32)
+        f, terms = fit.compute(genes, terms=True)
+    finally:
+        fit.close()
+    sel = [0, 64, 130, 199]
+    ref, ref_terms = _oracle_case(d).fitness(genes[sel], threads=16)
+    np.testing.assert_allclose(terms[sel], ref_terms, rtol=1e-5)
+    np.testing.assert_allclose(f[sel], ref, rtol=1e-5)
+
+
+@pytest.mark.parametrize("bf16,tol", [(0, 1e-5), (1, 3e-2)])
+def test_full_size_nn2b_matches_oracle(bf16, tol):
+    """The NN2B wurtzite workload at full size (800 x 72 atoms, n = 32,
+    1153 parameters, P = 256): two individuals against the oracle; FP32
+    tabulation within 1e-5 relative, the BF16 hidden layer within its stated
+    3e-2."""
+    import gpuga_amd
+    from gpuga_amd import synth
+    print_case("full size nn2b vs oracle", bf16)
+    d = _labelled(lambda: synth.wurtzite_ab(Nc=800, seed=5))
+    fit = gpuga_amd.Fitness(arrays=d, nn2b_bf16=bf16)
+    try:
+        genes = np.random.default_rng(80).random((256, fit.V), dtype=np.float32)
+        f = fit.compute(genes)
+    finally:
+        fit.close()
+    sel = [3, 255]
+    ref, _ = _oracle_case(d).fitness(genes[sel], threads=16)
+    rel = np.abs(f[sel] - ref) / np.abs(ref)
+    print("nn2b full size, bf16=%d: fitness rel error max %.3e" % (bf16, rel.max()))
+    assert rel.max() < tol
