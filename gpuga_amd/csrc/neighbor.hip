@@ -158,31 +158,46 @@ __global__ void __launch_bounds__(256) geo_triplets(int N, const int* NN, const 
 
 // Atom records of the Tersoff population kernel, the layout of
 // build_records: header (nn, species-pair mask, first bond, 0), M bonds,
-// M x M triplets, M reverse bonds, M inverse distances, zero-padded.
+// M x M triplets, M reverse bonds, M inverse distances, zero-padded.  One
+// thread per float4 of the record array, so that a wave's stores are
+// contiguous.
 __global__ void __launch_bounds__(256) geo_records(int N, int M, const int* NN, const int* start,
                                                    const int* rev, const int* ptype, const int* tri0,
                                                    const float4* bond, const float4* tri, float4* rec)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const int nn = NN[i], b0 = start[i];
   const long RS = 1 + M + (long)M * M + M / 2;
-  float4* R = rec + (long)i * RS;
-  int mask = 0;
-  for (int u = 0; u < nn; ++u) mask |= (ptype[b0 + u] & 1) << u;
-  R[0] = make_float4(__int_as_float(nn), __int_as_float(mask), __int_as_float(b0), 0.0f);
-  const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  for (int u = 0; u < M; ++u) R[1 + u] = u < nn ? bond[b0 + u] : zero;
-  const float4* T = tri + tri0[i];
-  for (int u = 0; u < M; ++u)
-    for (int w = 0; w < M; ++w)
-      R[1 + M + u * M + w] = u < nn && w < nn && u != w ? T[u * nn + w] : zero;
-  int* rv = reinterpret_cast<int*>(R + 1 + M + M * M);
-  float* inv = reinterpret_cast<float*>(R + 1 + M + M * M + M / 4);
-  for (int u = 0; u < M; ++u) {
-    rv[u] = u < nn ? rev[b0 + u] : 0;
-    inv[u] = u < nn ? (float)(1.0 / (double)bond[b0 + u].w) : 0.0f;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)N * RS) return;
+  const int i = (int)(e / RS), k = (int)(e - (long)i * RS);
+  const int nn = NN[i], b0 = start[i];
+  const int REV = 1 + M + M * M, INV = REV + M / 4;
+  float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (k == 0) {
+    int mask = 0;
+    for (int u = 0; u < nn; ++u) mask |= (ptype[b0 + u] & 1) << u;
+    v = make_float4(__int_as_float(nn), __int_as_float(mask), __int_as_float(b0), 0.0f);
+  } else if (k <= M) {
+    if (k - 1 < nn) v = bond[b0 + k - 1];
+  } else if (k < REV) {
+    const int u = (k - 1 - M) / M, w = (k - 1 - M) % M;
+    if (u < nn && w < nn && u != w) v = tri[tri0[i] + u * nn + w];
+  } else if (k < INV) {
+    int r[4];
+    for (int c = 0; c < 4; ++c) {
+      const int u = 4 * (k - REV) + c;
+      r[c] = u < nn ? rev[b0 + u] : 0;
+    }
+    v = make_float4(__int_as_float(r[0]), __int_as_float(r[1]), __int_as_float(r[2]),
+                    __int_as_float(r[3]));
+  } else {
+    float f[4];
+    for (int c = 0; c < 4; ++c) {
+      const int u = 4 * (k - INV) + c;
+      f[c] = u < nn ? (float)(1.0 / (double)bond[b0 + u].w) : 0.0f;
+    }
+    v = make_float4(f[0], f[1], f[2], f[3]);
   }
+  rec[e] = v;
 }
 
 #pragma clang fp contract(on)
@@ -255,8 +270,11 @@ void build_records_device(int N, int M, const DeviceNeighbors& d, const int* pty
 {
   if (N == 0) return;
   if (M != 4 && M != 8 && M != 16) fail("atom records hold 4, 8 or 16 bonds");
-  geo_records<<<(N + 255) / 256, 256, 0, st>>>(N, M, d.NN, d.start, d.rev, ptype, d.tri0, bond,
-                                               tri, rec);
+  const long total = (long)N * (1 + M + (long)M * M + M / 2);
+  const long blocks = (total + 255) / 256;
+  if (blocks > 0x7fffffffL) fail("atom record table too large");
+  geo_records<<<(unsigned)blocks, 256, 0, st>>>(N, M, d.NN, d.start, d.rev, ptype, d.tri0, bond,
+                                                tri, rec);
   GPUGA_HIP(hipGetLastError());
 }
 
