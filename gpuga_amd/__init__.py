@@ -84,6 +84,7 @@ def lib():
             "gpuga_fitness_set_timing": (C.c_int, [vp, C.c_int]),
             "gpuga_fitness_timing": (C.c_int, [vp, dp, P(C.c_long), C.c_int]),
             "gpuga_fitness_device_bytes": (C.c_int, [vp, dp]),
+            "gpuga_fitness_set_wait_limit": (C.c_int, [vp, C.c_int]),
             "gpuga_fitness_trace": (C.c_int, [vp, C.c_int, fp, dp, ip]),
             "gpuga_ga_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
                                           C.c_uint, P(vp)]),
@@ -264,6 +265,17 @@ class Fitness:
         _check(lib().gpuga_fitness_trace(self._h, g.shape[0], _ptr(g), _ptr(rows, C.c_double),
                                          C.byref(n)))
         return rows
+
+    def set_wait_limit(self, spins):
+        """Polls of the population kernel's dependency wait before a timeout
+        (< 0: time out at once; a test hook for the failure path)."""
+        _check(lib().gpuga_fitness_set_wait_limit(self._h, int(spins)))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def device_bytes(self):
         b = C.c_double()

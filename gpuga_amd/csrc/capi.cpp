@@ -372,6 +372,14 @@ int gpuga_fitness_trace(gpuga_fitness* h, int P, const float* genes, double* row
   });
 }
 
+int gpuga_fitness_set_wait_limit(gpuga_fitness* h, int spins)
+{
+  return guarded([&] {
+    need(h, "handle");
+    h->f->set_spin_limit(spins);
+  });
+}
+
 int gpuga_fitness_device_bytes(const gpuga_fitness* h, double* bytes)
 {
   return guarded([&] {
@@ -545,9 +553,41 @@ int gpuga_run(const char* input_dir, const gpuga_options* opt, double* seconds_p
     Fitness fit(dir, to_options(opt));
     GA ga(read_ga(dir), fit.number_of_variables(), fit.spec().pmin, fit.spec().pmax);
     const auto t0 = std::chrono::steady_clock::now();
-    ga.run(dir, [&](int P, const float* genes, float* fitness) {
-      fit.compute(P, genes, fitness);
-    });
+    const int P = ga.params().population_size, G = ga.params().maximum_generation;
+    if (fit.options().backend == 1 || P > kDeviceGaMaxP) {
+      // per-individual launches (or a population beyond the device GA):
+      // the host loop of GA::compute
+      ga.run(dir, [&](int n, const float* genes, float* fitness) {
+        fit.compute(n, genes, fitness);
+      });
+    } else {
+      // GA::compute (src/ga.cu:73-92) with the population and the GA step on
+      // the device, in runs of generations; the ga.out records of each run
+      // are written when it returns (the same lines GA::output writes)
+      const std::string path = dir + "/ga.out";
+      FILE* fid = std::fopen(path.c_str(), "w");
+      if (!fid) fail(strfmt("Failed to open %s!", path.c_str()));
+      const int V = ga.V(), run = 100;
+      std::vector<float> best(run), elite((size_t)run * V);
+      try {
+        for (int g = 0; g < G; g += run) {
+          const int n = std::min(run, G - g);
+          fit.run_ga(ga, g, n, best.data(), elite.data());
+          for (int i = 0; i < n; ++i) {
+            write_ga_line(fid, g + i, best[i], &elite[(size_t)i * V], fit.spec().pmin,
+                          fit.spec().pmax);
+            if ((g + i + 1) % 10 == 0)
+              write_ga_line(stdout, g + i + 1, best[i], &elite[(size_t)i * V], fit.spec().pmin,
+                            fit.spec().pmax);
+          }
+          std::fflush(fid);
+        }
+      } catch (...) {
+        std::fclose(fid);
+        throw;
+      }
+      std::fclose(fid);
+    }
     const auto t1 = std::chrono::steady_clock::now();
     if (seconds_per_generation)
       *seconds_per_generation = std::chrono::duration<double>(t1 - t0).count() /

@@ -701,6 +701,7 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   c.fitness = reinterpret_cast<float*>(c.terms + 3 * (size_t)P);
   c.status = reinterpret_cast<unsigned*>(c.fitness + P);
   c.gpart = pl.gpart.get();
+  c.spin_limit = spin_limit_;
   c.norm = norm();
   GPUGA_HIP(hipStreamSynchronize(st));
 }
@@ -836,6 +837,9 @@ void Fitness::run_ga(GA& ga, int first_generation, int generations, float* best,
   if (generations < 1) return;
   PopPlan& pl = plan(P);
   hipStream_t st = stream_;
+  // the GA as it was before this call: staging draws advances its random
+  // stream, so a failed run restores it
+  const GA before = ga;
   if (!pl.dga || !pl.dga->matches(ga)) {
     pl.drop_ga_graphs();
     pl.dga = std::make_unique<DeviceGA>(ga, pl.genes.get(), 1024, st, opt_.ga_batch);
@@ -884,9 +888,12 @@ void Fitness::run_ga(GA& ga, int first_generation, int generations, float* best,
       kernel_launches_ += n;
     }
     if (status != 0u) {
-      GPUGA_HIP(hipMemsetAsync(pl.ctl.status, 0, sizeof(unsigned), st));
-      GPUGA_HIP(hipStreamSynchronize(st));
-      fail("population kernel: force-assembly wait timed out; results discarded");
+      // the device population has advanced on invalid fitness values: drop
+      // the device GA, and the GA handle returns to its state before this call
+      ga = before;
+      pl.dga.reset();
+      pl.drop_ga_graphs();
+      discard_launch(pl);
     }
     dga.history(h.data(), n, st);
     for (int i = 0; i < n; ++i) {
@@ -897,6 +904,36 @@ void Fitness::run_ga(GA& ga, int first_generation, int generations, float* best,
     g += n;
   }
   dga.download(ga, generations & 1, st);
+}
+
+void Fitness::discard_launch(PopPlan& pl)
+{
+  // A dependency wait of the population kernel ran out: this launch's results
+  // are not valid.  A late phase-0 item may also have counted its chunk after
+  // the assembly items returned the counter to zero, so every counter and the
+  // result block (status word included) start from zero again before the
+  // plan is used next (the launch has ended: the stream is synchronised).
+  hipStream_t st = stream_;
+  GPUGA_HIP(hipStreamSynchronize(st));
+  pl.counters.zero(st);
+  pl.out.zero(st);
+  GPUGA_HIP(hipStreamSynchronize(st));
+  fail("population kernel: force-assembly wait timed out; results discarded");
+}
+
+void Fitness::set_spin_limit(int spins)
+{
+  spin_limit_ = spins;
+  for (auto& kv : plans_) {
+    PopPlan& pl = *kv.second;
+    pl.ctl.spin_limit = spins;
+    // the captured launches hold the old control block
+    pl.drop_ga_graphs();
+    if (pl.exec) (void)hipGraphExecDestroy(pl.exec);
+    if (pl.graph) (void)hipGraphDestroy(pl.graph);
+    pl.exec = nullptr;
+    pl.graph = nullptr;
+  }
 }
 
 void Fitness::compute_wait(float* fitness, double* terms)
@@ -915,13 +952,7 @@ void Fitness::compute_wait(float* fitness, double* terms)
   const double* h_terms = pl.h_out.get();
   const float* h_fitness = reinterpret_cast<const float*>(h_terms + 3 * (size_t)P);
   const unsigned h_status = *reinterpret_cast<const unsigned*>(h_fitness + P);
-  if (h_status != 0u) {
-    // a dependency wait of the population kernel ran out: the results of
-    // this launch are not valid; clear the flag and report
-    GPUGA_HIP(hipMemsetAsync(pl.ctl.status, 0, sizeof(unsigned), stream_));
-    GPUGA_HIP(hipStreamSynchronize(stream_));
-    fail("population kernel: force-assembly wait timed out; results discarded");
-  }
+  if (h_status != 0u) discard_launch(pl);
   if (fitness) std::memcpy(fitness, h_fitness, sizeof(float) * P);
   if (terms) std::memcpy(terms, h_terms, sizeof(double) * 3 * P);
 }
@@ -1083,7 +1114,10 @@ std::vector<double> Fitness::trace(int P, const float* genes)
   pl.ctl.trace = nullptr;
   std::vector<unsigned long long> t(4 * n);
   buf.download(t.data(), t.size(), st);
+  unsigned status = 0;
+  GPUGA_HIP(hipMemcpyAsync(&status, pl.ctl.status, sizeof status, hipMemcpyDeviceToHost, st));
   GPUGA_HIP(hipStreamSynchronize(st));
+  if (status != 0u) discard_launch(pl);
   int khz = 0;
   GPUGA_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_));
   const double ns_per_tick = 1e6 / std::max(khz, 1);

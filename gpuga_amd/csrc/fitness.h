@@ -135,6 +135,10 @@ public:
   void reset_timers() { kernel_ms_ = 0; kernel_launches_ = 0; }
   void set_timing(bool on) { timing_ = on; }
   FitnessOptions& options() { return opt_; }
+  // Polls of the population kernel's dependency wait before it reports a
+  // timeout (default about a second; < 0: report one at once, for tests of
+  // the failure path).
+  void set_spin_limit(int spins);
 
 private:
   void setup();
@@ -147,6 +151,9 @@ private:
   void launch_generation(PopPlan& pl, hipStream_t st); // graph replay or direct
   void capture(PopPlan& pl, hipStream_t st);           // the plan's graph, once
   double time_plan(int P, int chunk_cost, int reps);
+  // after a launch whose status word is set: clear the plan's counters and
+  // results, then fail
+  [[noreturn]] void discard_launch(PopPlan& pl);
   FitnessNorm norm() const;
   NN2BGrid nn2b_grid_desc() const;
 
@@ -169,6 +176,7 @@ private:
   DeviceBuffer<double> d_cerr_, d_terms1_;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   bool timing_ = false;
+  int spin_limit_ = kSpinLimit;
   double kernel_ms_ = 0.0;
   long kernel_launches_ = 0;
 };

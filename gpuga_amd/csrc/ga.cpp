@@ -246,16 +246,22 @@ void GA::mutation(int generation)
   for (int n = 0; n < count; ++n) population_[mut_idx_[n]] = mut_val_[n];
 }
 
-void GA::write_line(FILE* fid, int generation) const
+void write_ga_line(FILE* fid, int generation, float best, const float* genes,
+                   const std::vector<float>& pmin, const std::vector<float>& pmax)
 {
-  std::fprintf(fid, "%d %g ", generation, fitness_[0]);
-  for (int m = 0; m < V_; ++m) {
-    const float a = pmin_[m];
-    const float b = pmax_[m] - a;
-    const float bx = b * population_[m];
+  std::fprintf(fid, "%d %g ", generation, best);
+  for (size_t m = 0; m < pmin.size(); ++m) {
+    const float a = pmin[m];
+    const float b = pmax[m] - a;
+    const float bx = b * genes[m];
     std::fprintf(fid, "%g ", a + bx);
   }
   std::fprintf(fid, "\n");
+}
+
+void GA::write_line(FILE* fid, int generation) const
+{
+  write_ga_line(fid, generation, fitness_[0], population_.data(), pmin_, pmax_);
 }
 
 void GA::run(const std::string& dir, const std::function<void(int, const float*, float*)>& eval,

@@ -78,6 +78,11 @@ struct GAParams {
 GAParams read_ga(const std::string& dir);
 void validate_ga(const GAParams& p);
 
+// One ga.out line (src/ga.cu:138-147): generation, best fitness, then the
+// physical parameters a + (b - a) x of the elite genes x, each "%g ".
+void write_ga_line(FILE* fid, int generation, float best, const float* genes,
+                   const std::vector<float>& pmin, const std::vector<float>& pmax);
+
 class GA {
 public:
   GA(const GAParams& p, int V, const std::vector<float>& pmin, const std::vector<float>& pmax);

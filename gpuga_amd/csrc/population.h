@@ -93,6 +93,8 @@ struct PopCtl {
   unsigned* fetch;        // [nq] items handed out per queue, kFetchStride apart
   unsigned* done;         // [1] waves finished
   unsigned* status;       // [1] non-zero: a dependency wait timed out
+  int spin_limit;         // polls of a dependency wait before it gives up
+                          // (about 1 us each); < 0: give up at once
   float* fitness;         // [P]
   double* terms;          // [P][3]
   FitnessNorm norm;
@@ -135,10 +137,16 @@ __device__ inline unsigned wave_atomic_add(unsigned* p, unsigned x)
   return (unsigned)wave_first((int)old);
 }
 
-// Wait (bounded, about a second) until *p >= target; false on timeout.
-__device__ inline bool wave_wait_count(const unsigned* p, unsigned target)
+// Default poll count of a dependency wait (about a second).
+constexpr int kSpinLimit = 1 << 23;
+
+// Wait until *p >= target, polling at most `limit` more times after the
+// first look; false on timeout.  limit < 0 reports a timeout without
+// looking (the host sets it only to exercise the failure path).
+__device__ inline bool wave_wait_count(const unsigned* p, unsigned target, int limit)
 {
-  for (int spin = 0; spin < (1 << 23); ++spin) {
+  if (limit < 0) return false;
+  for (int spin = 0;; ++spin) {
     unsigned v = 0;
     if (__lane_id() == 0) v = cload(p);
     v = (unsigned)wave_first((int)v);
@@ -146,9 +154,9 @@ __device__ inline bool wave_wait_count(const unsigned* p, unsigned target)
       observe();
       return true;
     }
+    if (spin >= limit) return false;
     __builtin_amdgcn_s_sleep(8);
   }
-  return false;
 }
 
 // Sum n rows (stride `stride` doubles) of three arrays, 4 independent partial
@@ -379,7 +387,7 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
         if (lane == 0) *cnt = 0u;
       }
     } else {
-      if (!wave_wait_count(cnt, (unsigned)nch)) {
+      if (!wave_wait_count(cnt, (unsigned)nch, ctl.spin_limit)) {
         if (lane == 0) atomicOr(ctl.status, 1u);
       }
       if (packed) {

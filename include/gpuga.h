@@ -147,6 +147,14 @@ int gpuga_fitness_timing(gpuga_fitness* h, double* kernel_ms, long* launches, in
  * to the first start.  Call with rows = NULL to get *n_items. */
 int gpuga_fitness_trace(gpuga_fitness* h, int P, const float* genes, double* rows, int* n_items);
 
+/* Polls of the population kernel's dependency wait (a force-assembly item
+ * waiting for the partial forces of its configuration) before the launch
+ * reports a timeout: compute then fails, the launch's results are discarded
+ * and the handle stays usable.  Default about a second; < 0 reports a
+ * timeout at once (no reference counterpart: a test hook for the failure
+ * path, the reference has no dependency waits). */
+int gpuga_fitness_set_wait_limit(gpuga_fitness* h, int spins);
+
 /* Device bytes held by the handle (training set, tables, plans). */
 int gpuga_fitness_device_bytes(const gpuga_fitness* h, double* bytes);
 

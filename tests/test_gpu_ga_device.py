@@ -16,6 +16,19 @@ from conftest import print_case
 
 pytestmark = pytest.mark.gpu
 
+_OPEN = []
+
+
+def request_close(fit):
+    _OPEN.append(fit)
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _close_handles():
+    yield
+    while _OPEN:
+        _OPEN.pop().close()
+
 
 def fitness_seq(P, gens, seed, ties=False, nan_at=None):
     rng = np.random.default_rng(seed)
@@ -61,6 +74,7 @@ def test_run_ga_equals_host_loop(golden, name, P, parents):
     import gpuga_amd as G
     print_case("run_ga", name, P)
     fit = G.Fitness(golden(name))
+    request_close(fit)
     gens = 12
     host = G.GA(P, parents, fit.V, maximum_generation=gens, mutation_rate=0.2, seed=4242)
     dev = G.GA(P, parents, fit.V, maximum_generation=gens, mutation_rate=0.2, seed=4242)
@@ -92,6 +106,7 @@ def test_run_ga_across_staging_batches(golden, batch, gens):
     import gpuga_amd as G
     print_case("run_ga batches", "hfo2", 40, batch, gens)
     fit = G.Fitness(golden("hfo2"), ga_batch=batch)
+    request_close(fit)
     host = G.GA(40, 20, fit.V, maximum_generation=gens, mutation_rate=0.2, seed=99)
     dev = G.GA(40, 20, fit.V, maximum_generation=gens, mutation_rate=0.2, seed=99)
     best_h = []
@@ -101,3 +116,44 @@ def test_run_ga_across_staging_batches(golden, batch, gens):
     best_d, _ = fit.run_ga(dev, 0, gens)
     assert np.array_equal(np.array(best_h, np.float32), best_d)
     assert np.array_equal(host.population, dev.population)
+
+
+@pytest.mark.parametrize("P,parents,V,gens,ties", [(20, 10, 9, 30, False), (200, 100, 9, 20, True),
+                                                   (256, 128, 1153, 4, False),
+                                                   (1000, 500, 15, 5, True)])
+def test_device_step_equals_oracle(P, parents, V, gens, ties):
+    """The device GA step against the oracle's GA (oracle/ga.c, pinned to
+    libstdc++ and the reference's expressions) directly."""
+    import gpuga_amd as G
+    from oracle import oracle as O
+    print_case("device GA step vs oracle", P, parents, V)
+    dev = G.GA(P, parents, V, maximum_generation=gens, mutation_rate=0.2, seed=31337)
+    og = O.GA(P, parents, V, 31337, generations=gens, mutation_rate=0.2)
+    assert np.array_equal(dev.population, og.population)
+    for g, f in enumerate(fitness_seq(P, gens, 5 + P, ties)):
+        s = dev.step_device(g, f)
+        og.step(g, f)
+        assert np.array_equal(s, og.fitness), g
+        assert np.array_equal(dev.population, og.population), g
+
+
+@pytest.mark.parametrize("name,P,parents", [("si_diamond", 200, 100), ("nn2b", 64, 32)])
+def test_run_ga_equals_oracle_ga(golden, name, P, parents):
+    """gpuga_fitness_run_ga against the oracle's GA stepped on the product's
+    fitness of the oracle's population (the population launch is
+    deterministic: the same genes give the same fitness bit for bit)."""
+    import gpuga_amd as G
+    from oracle import oracle as O
+    print_case("run_ga vs oracle GA", name, P)
+    fit = G.Fitness(golden(name))
+    request_close(fit)
+    gens = 10
+    dev = G.GA(P, parents, fit.V, maximum_generation=gens, mutation_rate=0.2, seed=2718)
+    og = O.GA(P, parents, fit.V, 2718, generations=gens, mutation_rate=0.2)
+    best_o = []
+    for g in range(gens):
+        og.step(g, fit.compute(og.population))
+        best_o.append(og.fitness[0])
+    best_d, _ = fit.run_ga(dev, 0, gens)
+    assert np.array_equal(np.array(best_o, np.float32), best_d)
+    assert np.array_equal(dev.population, og.population)

@@ -21,7 +21,8 @@ def G():
 @pytest.mark.parametrize("name", ["si_diamond", "gaas", "si_type2", "nn2b", "hfo2"])
 def test_device_list_equals_oracle(G, golden, oracle_case, name):
     print_case("device neighbour list", name)
-    NN, start, NL = G.Fitness(golden(name)).neighbors()
+    with G.Fitness(golden(name)) as fit:
+        NN, start, NL = fit.neighbors()
     a, b, c = oracle_case(name).neighbors()
     assert np.array_equal(NN, a) and np.array_equal(start, b) and np.array_equal(NL, c)
 
@@ -51,7 +52,8 @@ def test_device_list_triclinic(G, seed):
     from oracle import oracle as O
     print_case("device neighbour list, triclinic", seed)
     d = triclinic_set(seed)
-    NN, start, NL = G.Fitness(arrays=d).neighbors()
+    with G.Fitness(arrays=d) as fit:
+        NN, start, NL = fit.neighbors()
     case = O.Case(potential=dict(potential_type=1, cutoff=3.0, weights=(0.05, 0.15, 0.8),
                                  pmin=d["pmin"], pmax=d["pmax"]), arrays=d)
     a, b, c = case.neighbors()
@@ -65,7 +67,8 @@ def test_device_list_large_set_equals_host_build(G, tmp_path):
     d = synth.tersoff_si(Nc=800, sizes="loguniform", seed=11)
     synth.write_dir(d, str(tmp_path), ga={"maximum_generation": 2, "population_size": 20,
                                           "parent_number": 10, "mutation_rate": 0.2})
-    NN, start, NL = G.Fitness(str(tmp_path)).neighbors()
+    with G.Fitness(str(tmp_path)) as fit:
+        NN, start, NL = fit.neighbors()
     a, b, c = G.inspect_neighbors(str(tmp_path))
     assert len(NN) > 90000
     assert np.array_equal(NN, a) and np.array_equal(start, b) and np.array_equal(NL, c)

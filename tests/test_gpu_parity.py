@@ -27,6 +27,14 @@ def G():
 _FIT = {}
 
 
+@pytest.fixture(autouse=True, scope="module")
+def _close_cached():
+    """The module's cached handles are closed when its tests are done."""
+    yield
+    while _FIT:
+        _FIT.popitem()[1].close()
+
+
 def product(G, golden, name, **opts):
     key = (name, tuple(sorted(opts.items())))
     if key not in _FIT:
@@ -143,8 +151,8 @@ def test_replay_is_bit_identical(G, golden):
         again, t2 = fit.compute(genes, terms=True)
         assert np.array_equal(first, again)
         assert np.array_equal(t1, t2)
-    eager = G.Fitness(golden("si_diamond"), use_graph=0, chunk_cost=fit_chunk(fit, 200))
-    e, te = eager.compute(genes, terms=True)
+    with G.Fitness(golden("si_diamond"), use_graph=0, chunk_cost=fit_chunk(fit, 200)) as eager:
+        e, te = eager.compute(genes, terms=True)
     np.testing.assert_allclose(te, t1, rtol=1e-12)
 
 
@@ -155,10 +163,10 @@ def fit_chunk(fit, P):
 def test_chunking_does_not_change_results(G, golden):
     print_case("chunking")
     genes = genes_for(9, 100, 33)
-    small = G.Fitness(golden("si_diamond"), chunk_cost=64)
-    whole = G.Fitness(golden("si_diamond"), chunk_cost=10 ** 8)
-    a, ta = small.compute(genes, terms=True)
-    b, tb = whole.compute(genes, terms=True)
+    with G.Fitness(golden("si_diamond"), chunk_cost=64) as small, \
+            G.Fitness(golden("si_diamond"), chunk_cost=10 ** 8) as whole:
+        a, ta = small.compute(genes, terms=True)
+        b, tb = whole.compute(genes, terms=True)
     np.testing.assert_allclose(ta, tb, rtol=1e-12)
 
 
@@ -169,15 +177,14 @@ def test_cross_wave_handoff_stress(G, golden, name):
     XCDs many times per launch; ten launches must reproduce the single-item
     result (FP64 partial sums regrouped: 1e-12)."""
     print_case("handoff stress", name)
-    V = G.Fitness(golden(name), chunk_cost=10 ** 8).V
-    genes = genes_for(V, 130, 77)
-    whole = G.Fitness(golden(name), chunk_cost=10 ** 8)
-    ref_fit, ref_terms = whole.compute(genes, terms=True)
-    tiny = G.Fitness(golden(name), chunk_cost=24, use_graph=1)
-    first = None
-    for rep in range(10):
-        f, t = tiny.compute(genes, terms=True)
-        np.testing.assert_allclose(t, ref_terms, rtol=1e-12)
-        if first is None:
-            first = t
-        assert np.array_equal(t, first)
+    with G.Fitness(golden(name), chunk_cost=10 ** 8) as whole, \
+            G.Fitness(golden(name), chunk_cost=24, use_graph=1) as tiny:
+        genes = genes_for(whole.V, 130, 77)
+        ref_fit, ref_terms = whole.compute(genes, terms=True)
+        first = None
+        for rep in range(10):
+            f, t = tiny.compute(genes, terms=True)
+            np.testing.assert_allclose(t, ref_terms, rtol=1e-12)
+            if first is None:
+                first = t
+            assert np.array_equal(t, first)
