@@ -64,12 +64,12 @@ def make_workload(name):
     return d, w
 
 
-def serve(args):
+def run_serve(args):
     """Served mixed workload: 16 fitting jobs (minimal Tersoff Si, AB2 rigid
     ion, NN2B wurtzite; 100-2000 configurations; populations 64-256) arrive
     as a Poisson process at 2 jobs/s and are served concurrently with
     training sets and populations resident on the GPU, admitted by a
-    device-memory budget."""
+    device-memory budget.  Returns the measurement as a dict."""
     import tempfile
 
     import gpuga_amd
@@ -116,8 +116,30 @@ def serve(args):
     wall = time.perf_counter() - t0
     lat = srv.latencies()
     adm = srv.admissions()
+    srv.close()
     gens = len(lat)
     work = sum(P * N * args.generations for _, _, P, N in jobs)
+    return dict(jobs=jobs, wall=wall, lat=lat, adm=adm, gens=gens, work=work)
+
+
+def serve_summary(r, args):
+    """The served workload's figures as they appear in the bench line."""
+    lat, adm = r["lat"], r["adm"]
+    return {"generations_per_s": r["gens"] / r["wall"], "generations": r["gens"],
+            "wall_s": r["wall"], "jobs": len(r["jobs"]), "arrival_rate_per_s": args.rate,
+            "generations_per_job": args.generations,
+            "p50_generation_ms": float(np.percentile(lat, 50)),
+            "p99_generation_ms": float(np.percentile(lat, 99)),
+            "p50_admission_ms": float(np.percentile(adm, 50)),
+            "p99_admission_ms": float(np.percentile(adm, 99)),
+            "individual_atom_evals_per_s": r["work"] / r["wall"],
+            "mix": [f"{k}:P{P}:N{N}" for _, k, P, N in r["jobs"]]}
+
+
+def serve(args):
+    r = run_serve(args)
+    jobs, wall, gens = r["jobs"], r["wall"], r["gens"]
+    sm = serve_summary(r, args)
     out = {
         "metric": "generations/s (served mixed workload)",
         "value": gens / wall,
@@ -131,14 +153,8 @@ def serve(args):
         "vs_baseline": None,
         "dtype": "f32 (f64 sums)",
         "data": "synthetic (seeded Si, AB2 fluorite and AB wurtzite cells)",
-        "config": {"workload": "serve", "jobs": len(jobs), "arrival_rate_per_s": args.rate,
-                   "generations_per_job": args.generations, "wall_s": wall,
-                   "p50_generation_ms": float(np.percentile(lat, 50)),
-                   "p99_generation_ms": float(np.percentile(lat, 99)),
-                   "p50_admission_ms": float(np.percentile(adm, 50)),
-                   "p99_admission_ms": float(np.percentile(adm, 99)),
-                   "individual_atom_evals_per_s": work / wall,
-                   "mix": [f"{k}:P{P}:N{N}" for _, k, P, N in jobs]},
+        "config": dict(workload="serve", **{k: v for k, v in sm.items()
+                                             if k not in ("generations_per_s", "generations")}),
     }
     print(json.dumps(out), flush=True)
 
@@ -166,10 +182,83 @@ def cpu_baseline(d, w, genes, budget_s=12.0):
     case.fitness(genes[:m], threads=threads)
     dtm = time.perf_counter() - t0
     return {"value": n * N / dt, "unit": "individual*atom evals/s", "cores": 1, "kind": "port",
+            "cpu": cpu_model(),
             "sample": f"{n} individuals x {N} atoms (full training set), FP64 scalar C oracle, "
                       f"1 thread, {dt:.1f} s",
             "openmp": {"value": m * N / dtm, "cores": threads,
                        "sample": f"{m} individuals x {N} atoms, {threads} threads, {dtm:.1f} s"}}
+
+
+def plain_leg(fit, ga, steps, batched_ms):
+    """The per-individual-launch form of the same generation on the same GPU
+    (one find_force, one reduction and one host synchronisation per
+    individual, then the host GA step), timed over `steps` generations after
+    an untimed one, against the batched generation time of this run."""
+    import gpuga_amd
+    P, V = ga.population.shape
+    g = gpuga_amd.GA(P, ga.parent_number, V, maximum_generation=steps + 1, mutation_rate=0.2,
+                     seed=4242)
+    g.step(0, fit.compute(g.population, plain=True))
+    t0 = time.perf_counter()
+    for s in range(1, steps + 1):
+        g.step(s, fit.compute(g.population, plain=True))
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    return {"ms_per_generation": ms, "generations": steps,
+            "individual_atom_evals_per_s": P * fit.N / (ms * 1e-3),
+            "batched_ms_per_generation": batched_ms, "speedup_batched_over_plain": ms / batched_ms}
+
+
+def trace_split(fit, genes, n_force_cfg):
+    """Where the population launch's wave time goes, from one generation's
+    work-item timeline (gpuga_fitness_trace): per resident wave, the time in
+    phase-0 items of force and of energy configurations and in
+    force-assembly items (an item's time ends when the wave has fetched its
+    next item, so it includes that fetch), before its first item (entry and
+    first fetch) and idle after its last one, as fractions of waves x launch
+    span."""
+    rows = fit.trace(genes)
+    c, k, a0, a1, ph, slot, t0, t1, entry = (rows[:, 0], rows[:, 1], rows[:, 3], rows[:, 4],
+                                            rows[:, 5], rows[:, 6], rows[:, 7], rows[:, 8],
+                                            rows[:, 9])
+    begin = float(np.nanmin(np.where(np.isnan(entry), t0, entry)))
+    end = float(t1.max())
+    span = end - begin
+    waves = np.unique(slot)
+    dur = t1 - t0
+    force = c < n_force_cfg
+    cls = {"phase0_force": (ph == 0) & force, "phase0_energy": (ph == 0) & ~force,
+           "phase1_assembly": ph == 1}
+    total = span * len(waves)
+    out = {"span_us": span / 1e3, "waves": int(len(waves)), "items": int(len(rows))}
+    for name, m in cls.items():
+        atoms = float(np.sum(a1[m] - a0[m]))
+        out[name] = {"frac": float(dur[m].sum() / total), "items": int(m.sum()),
+                     "mean_item_us": float(dur[m].mean() / 1e3) if m.any() else 0.0,
+                     "us_per_atom_step": float(dur[m].sum() / max(atoms, 1) / 1e3)}
+    head = tail = 0.0
+    for w in waves:
+        m = slot == w
+        head += float(t0[m].min()) - begin
+        tail += end - float(t1[m].max())
+    out["head_frac"] = head / total
+    out["tail_frac"] = tail / total
+    ends = np.sort([float(t1[slot == w].max()) - begin for w in waves])
+    out["wave_end_us"] = {"p10": float(np.percentile(ends, 10) / 1e3),
+                          "p50": float(np.percentile(ends, 50) / 1e3),
+                          "p90": float(np.percentile(ends, 90) / 1e3),
+                          "max": float(ends[-1] / 1e3)}
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def measured_traffic(workload):
@@ -204,6 +293,12 @@ def main():
                     help="GA step on the host each generation (genes and fitness cross PCIe)")
     ap.add_argument("--chunk-cost", type=int, default=0,
                     help="work-item size; 0 = autotune before timing")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="headline line without the served-workload leg")
+    ap.add_argument("--plain-steps", type=int, default=3,
+                    help="generations of the per-individual-launch leg (0: none)")
+    ap.add_argument("--trace-split", action="store_true",
+                    help="add the work-item timeline split of one population launch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the population-sharded multi-GPU form "
@@ -310,9 +405,22 @@ def main():
     }
     if note:
         out["roofline"]["note"] = note
+    if out["roofline"]["traffic"] is not None:
+        out["roofline"]["traffic_source"] = (
+            "rocprofv3 FETCH_SIZE + WRITE_SIZE of this workload's population kernel, "
+            "each scaled by the counter's reading for a copy of known size "
+            "(profiles/traffic.json; not measured in this run)")
+    if args.trace_split and not args.plain:
+        out["trace_split"] = trace_split(fit, ga.population, fit.Nc_force)
+    if device_ga and args.plain_steps > 0:
+        out["plain"] = plain_leg(fit, ga, args.plain_steps, ms_step)
+    fit.close()
     if not args.no_cpu_baseline and not args.plain:
         genes = synth.random_genes(256, V, seed=99)
         out["cpu_baseline"] = cpu_baseline(d, w, genes)
+    if args.workload == "tersoff_batched" and device_ga and not args.no_serve:
+        r = run_serve(args)
+        out["serve"] = serve_summary(r, args)
     print(json.dumps(out), flush=True)
 
 

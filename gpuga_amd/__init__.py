@@ -303,6 +303,7 @@ class GA:
         _check(lib().gpuga_ga_create(population_size, parent_number, V, maximum_generation,
                                      mutation_rate, seed, C.byref(self._h)))
         self.P, self.V = population_size, V
+        self.parent_number = parent_number
 
     @property
     def population(self):
@@ -451,9 +452,19 @@ class Server:
         _check(fn(self._h, _ptr(out, C.c_double), C.byref(n)))
         return out[: n.value]
 
+    def close(self):
+        if getattr(self, "_h", None) and _LIB is not None and self._h.value:
+            _LIB.gpuga_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def __del__(self):
         try:
-            if self._h.value:
-                lib().gpuga_server_destroy(self._h)
+            self.close()
         except Exception:
             pass

@@ -96,7 +96,7 @@ def test_cli_runs_jobs_in_turn_and_reports_errors(golden, tmp_path):
     for name in ("hfo2", "si_diamond"):
         d = str(tmp_path / name)
         shutil.copytree(golden(name), d)
-        _write_ga_in(d, 3, 12, 6)
+        _write_ga_in(d, 3, 20, 10)
         dirs.append(d)
     out = _run_cli(tmp_path, dirs)
     assert out.count("Run simulation for") == 2

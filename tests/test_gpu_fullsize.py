@@ -108,7 +108,8 @@ def test_full_size_rigid_ion_matches_oracle():
 @pytest.mark.parametrize("bf16,tol", [(0, 1e-5), (1, 3e-2)])
 def test_full_size_nn2b_matches_oracle(bf16, tol):
     """The NN2B wurtzite workload at full size (800 x 72 atoms, n = 32,
-    1153 parameters, P = 256): two individuals against the oracle; FP32
+    1153 parameters, P = 256): one individual of every lane block against the
+    oracle, fitness and each of the three terms (force included); FP32
     tabulation within 1e-5 relative, the BF16 hidden layer within its stated
     3e-2."""
     import gpuga_amd
@@ -118,11 +119,14 @@ def test_full_size_nn2b_matches_oracle(bf16, tol):
     fit = gpuga_amd.Fitness(arrays=d, nn2b_bf16=bf16)
     try:
         genes = np.random.default_rng(80).random((256, fit.V), dtype=np.float32)
-        f = fit.compute(genes)
+        f, terms = fit.compute(genes, terms=True)
     finally:
         fit.close()
-    sel = [3, 255]
-    ref, _ = _oracle_case(d).fitness(genes[sel], threads=16)
+    sel = [3, 70, 140, 255]  # lane blocks 0, 1, 2, 3
+    ref, ref_terms = _oracle_case(d).fitness(genes[sel], threads=16)
     rel = np.abs(f[sel] - ref) / np.abs(ref)
-    print("nn2b full size, bf16=%d: fitness rel error max %.3e" % (bf16, rel.max()))
+    trel = np.abs(terms[sel] - ref_terms) / np.abs(ref_terms)
+    print("nn2b full size, bf16=%d: fitness rel error max %.3e, terms (E, W, F) %s"
+          % (bf16, rel.max(), np.array2string(trel.max(axis=0), precision=3)))
     assert rel.max() < tol
+    assert trel.max() < tol
