@@ -35,10 +35,20 @@ MI355X_HBM_PEAK = 8.0e12     # B/s
 # Algorithmic work per individual x atom evaluation (4 neighbours per atom):
 # pair terms ~106 FLOP x 4, triplets ~57 FLOP x 12, assembly -> ~1.1 kFLOP.
 TERSOFF_FLOP_PER_UNIT = 1.1e3
-RI_FLOP_PER_UNIT_PER_NEIGHBOR = 100.0
-# NN2B (n = 32): 4 n^2 = 4096 MFMA FLOP + ~520 VALU FLOP per ordered pair
-NN2B_FLOP_PER_PAIR = 4096.0 + 520.0
-MI355X_BF16_PEAK = 2.5e15  # FLOP/s, dense BF16 MFMA
+# Pair potentials: the work per pair the population kernels evaluate (each
+# unordered pair once in energy/virial configurations, every ordered pair in
+# force configurations; distance-only factors such as erfc(a r)/r come from
+# the set-up tables).  FMA = 2 FLOP; exp2 not counted.
+#  RI, per pair: Buckingham + shifted-force Coulomb value and derivative
+#  (14 FLOP), + energy and virial accumulation in energy configurations
+#  (31 FLOP in all); a Morse term on unlike pairs adds 10.
+RI_FLOP_FORCE_PAIR, RI_FLOP_ENERGY_PAIR, RI_FLOP_MORSE = 14.0, 31.0, 10.0
+#  NN2B: per individual the 1-n-n-1 network with its first two derivatives
+#  on every grid point of the tabulation (6 n^2 + 23 n FLOP), then per pair
+#  the Hermite reconstruction of p and p' and the taper (18 FLOP) plus the
+#  force (6) or energy and virial (17) accumulation.
+NN2B_FLOP_FORCE_PAIR, NN2B_FLOP_ENERGY_PAIR = 24.0, 35.0
+NN2B_GRID = 1024
 
 WORKLOADS = {
     "tersoff_batched": dict(kind="tersoff", Nc=1000, sizes="loguniform", P=200, seed=2),
@@ -189,6 +199,20 @@ def cpu_baseline(d, w, genes, budget_s=12.0):
                        "sample": f"{m} individuals x {N} atoms, {threads} threads, {dtm:.1f} s"}}
 
 
+def pair_counts(fit, d):
+    """Pairs a pair-potential population kernel evaluates per individual, by
+    pair type t_i + t_j: every bond of a force-configuration atom, each
+    unordered pair of the other configurations."""
+    NN, start, NL = fit.neighbors()
+    typ = np.asarray(d["type"])
+    owner = np.repeat(np.arange(fit.N), NN)
+    pt = typ[owner] + typ[NL]
+    force = owner < fit.N_force
+    fp = np.bincount(pt[force], minlength=3)[:3]
+    ep = np.bincount(pt[~force & (NL > owner)], minlength=3)[:3]
+    return fp.astype(np.float64), ep.astype(np.float64)
+
+
 def plain_leg(fit, ga, steps, batched_ms):
     """The per-individual-launch form of the same generation on the same GPU
     (one find_force, one reduction and one host synchronisation per
@@ -242,6 +266,22 @@ def trace_split(fit, genes, n_force_cfg):
         tail += end - float(t1[m].max())
     out["head_frac"] = head / total
     out["tail_frac"] = tail / total
+    # item time against its atom count (a least-squares line per class)
+    steps = a1 - a0
+    for name, m in cls.items():
+        if m.sum() > 2:
+            A = np.stack([np.ones(m.sum()), steps[m]], 1)
+            coef = np.linalg.lstsq(A, dur[m], rcond=None)[0]
+            out[name]["fit_us"] = {"per_item": float(coef[0] / 1e3),
+                                   "per_atom_step": float(coef[1] / 1e3)}
+    last = np.argsort(t1)[-12:]
+    out["last_items"] = [{"c": int(c[i]), "k": int(k[i]), "phase": int(ph[i]),
+                          "atoms": int(a1[i] - a0[i]), "start_us": float((t0[i] - begin) / 1e3),
+                          "end_us": float((t1[i] - begin) / 1e3)} for i in last]
+    starts = np.sort(t0 - begin)
+    out["item_start_us"] = {"p90": float(np.percentile(starts, 90) / 1e3),
+                            "p99": float(np.percentile(starts, 99) / 1e3),
+                            "max": float(starts[-1] / 1e3)}
     ends = np.sort([float(t1[slot == w].max()) - begin for w in waves])
     out["wave_end_us"] = {"p10": float(np.percentile(ends, 10) / 1e3),
                           "p50": float(np.percentile(ends, 50) / 1e3),
@@ -359,21 +399,29 @@ def main():
     ms_step = 1e3 * elapsed / args.steps
     value = P * N * args.steps / elapsed
     kernel_s = kms * 1e-3 / max(launches, 1)
-    # Algorithmic work of the reference formulation per generation (the
-    # per-unit counts above, FMA = 2 FLOP); the population kernels skip
-    # part of it (precomputed geometry, unordered pairs, tabulated NN2B), so
-    # for the pair potentials the fraction can exceed what the VALU executes.
+    # Algorithmic work per generation (the per-unit counts above, FMA = 2
+    # FLOP) over the generation's kernel time.
+    peak, bound = MI355X_FP32_PEAK, "valu"
     if w["kind"] == "tersoff":
-        flop, peak, bound = TERSOFF_FLOP_PER_UNIT * P * N, MI355X_FP32_PEAK, "valu"
+        flop = TERSOFF_FLOP_PER_UNIT * P * N
         note = None
-    elif w["kind"] == "ri":
-        flop, peak, bound = RI_FLOP_PER_UNIT_PER_NEIGHBOR * fit.bonds * P, MI355X_FP32_PEAK, "valu"
-        note = ("reference count: every ordered pair at ~100 FLOP; energy configurations "
-                "evaluate each unordered pair once here and erfc/geometry are precomputed")
     else:
-        flop, peak, bound = NN2B_FLOP_PER_PAIR * fit.bonds * P, MI355X_BF16_PEAK, "mfma"
-        note = ("reference count: the 1-n-n-1 network per ordered pair; here it runs once "
-                "per grid point and individual (tabulation), pairs interpolate")
+        fp, ep = pair_counts(fit, d)
+        if w["kind"] == "ri":
+            # the Morse term (parameter 11, D0) runs on unlike pairs when any
+            # individual can have one
+            morse = RI_FLOP_MORSE if (d["pmin"][11] != 0 or d["pmax"][11] != 0) else 0.0
+            flop = P * (RI_FLOP_FORCE_PAIR * fp.sum() + RI_FLOP_ENERGY_PAIR * ep.sum() +
+                        morse * (fp[1] + ep[1]))
+            note = (f"{int(fp.sum())} ordered force-configuration pairs, {int(ep.sum())} "
+                    "unordered energy-configuration pairs per individual")
+        else:
+            n = fit.num_neurons
+            flop = P * (NN2B_FLOP_FORCE_PAIR * fp.sum() + NN2B_FLOP_ENERGY_PAIR * ep.sum() +
+                        NN2B_GRID * (6.0 * n * n + 23.0 * n))
+            note = (f"{int(fp.sum())} ordered force-configuration pairs, {int(ep.sum())} "
+                    f"unordered energy-configuration pairs and a {NN2B_GRID}-point tabulation "
+                    "per individual; time = tabulation + pair kernel")
     if args.plain:
         flop /= P  # timing covers one individual per launch group
     achieved = flop / kernel_s

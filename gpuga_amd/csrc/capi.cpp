@@ -322,8 +322,21 @@ int gpuga_inspect_geometry(const char* dir, int* sizes, float* bond, float* tri,
         t.resize(g.tri.size());
         std::memcpy(t.data(), g.tri.data(), sizeof(float4) * g.tri.size());
       }
-      if (const int M = record_slots(nl.max_nn)) {
-        const std::vector<Vec4> rv = build_records(nl, g, bond_types(ts, nl, true), M);
+      // the population kernel's records: bonds shorter than the largest R2
+      // (the whole list when none is that long)
+      const float reach = tersoff_reach(s);
+      NeighborList pn;
+      if (reach < s.cutoff && nl.total > 0) {
+        std::vector<float> d(nl.total);
+        for (int k = 0; k < nl.total; ++k) d[k] = g.bond[k].w;
+        pn = prune_neighbors(nl, d, reach);
+      }
+      const bool pruned = pn.total < nl.total && pn.NN.size() == nl.NN.size();
+      const NeighborList& rl = pruned ? pn : nl;
+      if (const int M = record_slots(rl.max_nn)) {
+        const Geometry pg = pruned ? build_geometry(ts, rl, true) : Geometry();
+        const std::vector<Vec4> rv =
+          build_records(rl, pruned ? pg : g, bond_types(ts, rl, true), M);
         r.resize(rv.size());
         std::memcpy(r.data(), rv.data(), sizeof(float4) * rv.size());
       }

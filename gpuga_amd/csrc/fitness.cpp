@@ -9,6 +9,10 @@
 namespace gpuga {
 
 int tersoff_population_occupancy(int cls, bool two, bool pack, int block);
+
+// Neighbours within R2 the looped Tersoff population kernel takes (its LDS
+// rows: 4 x 64 lanes x 4 B per neighbour and wave, 64 KiB at this bound).
+constexpr int kLoopMaxNeighbors = 64;
 int ri_population_occupancy(int block);
 int nn2b_population_occupancy(int block);
 
@@ -43,10 +47,15 @@ public:
   {
     q_ = tersoff_params(num_types_, p.data());
   }
-  void find_force(const DevSet& s, float* f, float* virial, float* pe, hipStream_t st) override
+  void find_force(const DevSet& s, float* f, float* virial, float* pe, hipStream_t st,
+                  float* ws) override
   {
-    launch_tersoff_plain(s, q_, b_.get(), bp_.get(), f12_.get(), f, virial, pe, st);
+    // workspace: b and b' per bond, then f_ij (3 per bond)
+    const size_t B = b_.size();
+    if (ws) launch_tersoff_plain(s, q_, ws, ws + B, ws + 2 * B, f, virial, pe, st);
+    else launch_tersoff_plain(s, q_, b_.get(), bp_.get(), f12_.get(), f, virial, pe, st);
   }
+  size_t workspace_floats() const override { return 5 * b_.size(); }
 
 private:
   int num_types_;
@@ -59,7 +68,8 @@ public:
   explicit RIPotential(const float4* tab) : tab_(tab) {}
   void initialize(int, int) override {}
   void update_potential(const std::vector<float>& p) override { q_ = ri_params(p.data()); }
-  void find_force(const DevSet& s, float* f, float* virial, float* pe, hipStream_t st) override
+  void find_force(const DevSet& s, float* f, float* virial, float* pe, hipStream_t st,
+                  float*) override
   {
     launch_ri_plain(s, q_, tab_, f, virial, pe, st);
   }
@@ -78,7 +88,8 @@ public:
     host_ = p;
     dirty_ = true;
   }
-  void find_force(const DevSet& s, float* f, float* virial, float* pe, hipStream_t st) override
+  void find_force(const DevSet& s, float* f, float* virial, float* pe, hipStream_t st,
+                  float*) override
   {
     if (dirty_) {
       w_.upload(host_, st);
@@ -120,7 +131,8 @@ size_t DeviceSet::bytes() const
   return Na.bytes() + Na_sum.bytes() + type.bytes() + nn.bytes() + bond0.bytes() + nl.bytes() +
          rev.bytes() + ptype.bytes() + tri0.bytes() + bond.bytes() + tri.bytes() +
          ri_tab.bytes() + rec.bytes() + pair_hdr.bytes() + pair_tab.bytes() + pe_ref.bytes() +
-         virial_ref.bytes() + force_ref.bytes();
+         virial_ref.bytes() + force_ref.bytes() + p_nn.bytes() + p_b0.bytes() + p_nl.bytes() +
+         p_rev.bytes() + p_ptype.bytes() + p_tri0.bytes() + p_bond.bytes() + p_tri.bytes();
 }
 
 void PopPlan::drop_ga_graphs()
@@ -228,15 +240,81 @@ void Fitness::setup()
     else ds_.tri.resize((size_t)tri_total);
     dn.tri0 = ds_.tri0.get();
     build_geometry_device(ts_, dn, ds_.bond.get(), ds_.tri.get(), st);
-    if (const int M = record_slots(nl_.max_nn)) {
+    // The population kernel's records leave out the bonds that no
+    // individual's taper reaches (exact zeros in every sum).
+    const float reach = tersoff_reach(spec_);
+    pruned_ = false;
+    if (reach < spec_.cutoff && nl_.total > 0) {
+      std::vector<float4> b(nl_.total);
+      ds_.bond.download(b.data(), b.size(), st);
+      GPUGA_HIP(hipStreamSynchronize(st));
+      std::vector<float> d(nl_.total);
+      for (int k = 0; k < nl_.total; ++k) d[k] = b[k].w;
+      pnl_ = prune_neighbors(nl_, d, reach);
+      pruned_ = pnl_.total < nl_.total;
+      if (!pruned_) pnl_ = NeighborList();
+    }
+    const NeighborList& pn = population_neighbors();
+    const int M = record_slots(pn.max_nn);
+    loop_path_ = M == 0;
+    if (loop_path_ && pn.max_nn > kLoopMaxNeighbors)
+      fail(strfmt("the population kernel supports at most %d neighbours per atom within the "
+                  "largest R2 (found %d); use the per-individual backend",
+                  kLoopMaxNeighbors, pn.max_nn));
+    // tables of the list the population kernel reads: the full ones, or
+    // the pruned list's (geometry bit-identical to the full tables' entries
+    // for the same bonds)
+    DeviceNeighbors pd = dn;
+    const int* pptype = ds_.ptype.get();
+    const float4 *pbond = ds_.bond.get(), *ptri = ds_.tri.get();
+    if (pruned_) {
+      long long ptri_total = 0;
+      const std::vector<int> t0 = triplet_offsets(pn, &ptri_total);
+      ds_.p_nn.upload(pn.NN, st);
+      ds_.p_b0.upload(pn.start, st);
+      ds_.p_nl.upload(pn.NL.empty() ? std::vector<int>{0} : pn.NL, st);
+      ds_.p_rev.upload(pn.rev.empty() ? std::vector<int>{0} : pn.rev, st);
+      const std::vector<int> pt = bond_types(ts_, pn, true);
+      ds_.p_ptype.upload(pt.empty() ? std::vector<int>{0} : pt, st);
+      ds_.p_tri0.upload(t0, st);
+      ds_.p_bond.resize(std::max(pn.total, 1));
+      ds_.p_tri.resize((size_t)std::max<long long>(ptri_total, 1));
+      pd.NN = ds_.p_nn.get();
+      pd.start = ds_.p_b0.get();
+      pd.NL = ds_.p_nl.get();
+      pd.rev = ds_.p_rev.get();
+      pd.tri0 = ds_.p_tri0.get();
+      pd.total = pn.total;
+      build_geometry_device(ts_, pd, ds_.p_bond.get(), ds_.p_tri.get(), st);
+      pptype = ds_.p_ptype.get();
+      pbond = ds_.p_bond.get();
+      ptri = ds_.p_tri.get();
+      // (the host vectors above are read before this scope ends)
+      GPUGA_HIP(hipStreamSynchronize(st));
+    }
+    if (M > 0) {
       const size_t RS = 1 + M + (size_t)M * M + M / 2;
       ds_.rec.resize((size_t)ts_.N * RS);
-      build_records_device(ts_.N, M, dn, ds_.ptype.get(), ds_.bond.get(), ds_.tri.get(),
-                           ds_.rec.get(), st);
+      build_records_device(ts_.N, M, pd, pptype, pbond, ptri, ds_.rec.get(), st);
       ds_.view.rec_nmax = M;
-      ds_.view.bond_force =
-        ts_.N_force > 0 ? nl_.start[ts_.N_force - 1] + nl_.NN[ts_.N_force - 1] : 0;
+      GPUGA_HIP(hipStreamSynchronize(st));
+      // the records hold everything the kernel reads of the pruned list
+      for (auto* b : {&ds_.p_nn, &ds_.p_b0, &ds_.p_nl, &ds_.p_rev, &ds_.p_ptype, &ds_.p_tri0})
+        b->resize(0);
+      ds_.p_bond.resize(0);
+      ds_.p_tri.resize(0);
+    } else {
+      loop_.nn = pd.NN;
+      loop_.b0 = pd.start;
+      loop_.tri0 = pd.tri0;
+      loop_.rev = pd.rev;
+      loop_.ptype = pptype;
+      loop_.bond = pbond;
+      loop_.tri = ptri;
+      loop_.max_nn = pn.max_nn;
     }
+    ds_.view.bond_force =
+      ts_.N_force > 0 ? pn.start[ts_.N_force - 1] + pn.NN[ts_.N_force - 1] : 0;
     dmin_ = dmax_ = 1.0f; // (the NN2B tabulation range; unused here)
   } else {
     ds_.tri0.upload(std::vector<int>(ts_.N, 0), st);
@@ -456,8 +534,9 @@ int nn_class(int max_nn)
 size_t Fitness::plan_bytes(int P) const
 {
   const size_t K = (P + WAVE - 1) / WAVE;
-  const size_t B_force = ts_.N_force > 0 ? (size_t)nl_.start[ts_.N_force - 1] +
-                                             nl_.NN[ts_.N_force - 1]
+  const NeighborList& pn = population_neighbors();
+  const size_t B_force = ts_.N_force > 0 ? (size_t)pn.start[ts_.N_force - 1] +
+                                             pn.NN[ts_.N_force - 1]
                                          : 0;
   const bool tersoff = spec_.type == POT_TERSOFF1 || spec_.type == POT_TERSOFF2;
   size_t b = K * 3 * ts_.Nc * WAVE * sizeof(double);              // err
@@ -471,11 +550,11 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
 {
   const int Nc = ts_.Nc;
   const bool tersoff = spec_.type == POT_TERSOFF1 || spec_.type == POT_TERSOFF2;
-  if (tersoff && nl_.max_nn > 16)
-    fail(strfmt("population kernel supports at most 16 neighbours per atom (found %d); "
-                "use the per-individual backend",
-                nl_.max_nn));
+  // neighbour counts of the list the kernel reads (Tersoff: the pruned one)
+  const NeighborList& pn = tersoff ? population_neighbors() : nl_;
   pl.P = P;
+  // the looped Tersoff kernel: one wave per workgroup (its LDS rows)
+  pl.block = tersoff && loop_path_ ? WAVE : 256;
   pl.K = (P + WAVE - 1) / WAVE;
   pl.chunk_cost = chunk_cost;
   const int K = pl.K;
@@ -491,7 +570,7 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
     const int a0 = ts_.Na_sum[c], a1 = a0 + ts_.Na[c];
     const bool fc = c < ts_.Nc_force;
     long long total = 0;
-    for (int i = a0; i < a1; ++i) total += atom_cost(spec_.type, nl_.NN[i], fc);
+    for (int i = a0; i < a1; ++i) total += atom_cost(spec_.type, pn.NN[i], fc);
     int n = (int)std::max<long long>(1, (total + chunk_cost - 1) / std::max(chunk_cost, 1));
     n = std::min(n, a1 - a0);
     nch[c] = n;
@@ -501,10 +580,10 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
     int start = a0, k = 0, mx = 0;
     long long cacc = 0;
     for (int i = a0; i < a1; ++i) {
-      const long long ci = atom_cost(spec_.type, nl_.NN[i], fc);
+      const long long ci = atom_cost(spec_.type, pn.NN[i], fc);
       acc += ci;
       cacc += ci;
-      mx = std::max(mx, nl_.NN[i]);
+      mx = std::max(mx, pn.NN[i]);
       // cut at the k-th cost quantile, or when the atoms left are exactly
       // the chunks still to make
       const int left_atoms = a1 - (i + 1), left_chunks = n - 1 - k;
@@ -537,7 +616,7 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   // G = 2, P = 216, the launch took 0.607 ms packed against 0.521 ms
   // unpacked; with G = 4, P = 208, 0.458 against 0.523 ms.)
   int G_last = 1;
-  if (tersoff && opt_.pack_lanes) {
+  if (tersoff && opt_.pack_lanes && !loop_path_) {
     const int Pr = P - (K - 1) * WAVE;
     int W = 1;
     while (W < Pr) W <<= 1;
@@ -578,8 +657,10 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
       case POT_RI: occ = ri_population_occupancy(pl.block); break;
       case POT_NN2B: occ = nn2b_population_occupancy(pl.block); break;
       default:
-        occ = tersoff_population_occupancy(nn_class(nl_.max_nn), spec_.type == POT_TERSOFF2,
-                                           G_last > 1, pl.block);
+        occ = loop_path_ ? tersoff_loop_occupancy(spec_.type == POT_TERSOFF2, pn.max_nn, pl.block)
+                         : tersoff_population_occupancy(nn_class(pn.max_nn),
+                                                        spec_.type == POT_TERSOFF2, G_last > 1,
+                                                        pl.block);
         break;
       }
       occ = std::max(occ, 1);
@@ -659,7 +740,7 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   pl.out.zero(st);
   pl.genes.resize((size_t)P * spec_.V);
   const int B_force =
-    ts_.N_force > 0 ? nl_.start[ts_.N_force - 1] + nl_.NN[ts_.N_force - 1] : 0;
+    ts_.N_force > 0 ? pn.start[ts_.N_force - 1] + pn.NN[ts_.N_force - 1] : 0;
   pl.bondf.resize(tersoff ? std::max<size_t>((size_t)K * (B_force + ts_.N_force) * 3 * WAVE, 1) : 1);
   if (spec_.type == POT_NN2B) pl.nn2b_table.resize((size_t)K * nn2b_grid_desc().G * 3 * WAVE);
   const int groups = (Nc + CFG_GROUP - 1) / CFG_GROUP;
@@ -739,17 +820,33 @@ void Fitness::launch_population(PopPlan& pl, hipStream_t st, const float* genes)
     break;
   }
   default: {
+    if (loop_path_) {
+      launch_tersoff_loop_population(ctl, ds_.view, loop_, pl.grid, pl.block, st);
+      break;
+    }
     // the instantiation with the packed path only when the plan has packed items
-    launch_tersoff_population(ctl, ds_.view, nn_class(nl_.max_nn), pl.grid, pl.block,
+    launch_tersoff_population(ctl, ds_.view, nn_class(population_neighbors().max_nn), pl.grid,
+                              pl.block,
                               pl.groups_last > 1, st);
     break;
   }
   }
 }
 
+void Fitness::check_genes(int P, const float* genes) const
+{
+  // the pruned records leave out bonds beyond R2 at its upper bound, which
+  // bounds R2 for genes in [0, 1] only
+  if (!pruned_) return;
+  for (int p = 0; p < P; ++p)
+    if (!(genes[(size_t)p * spec_.V + spec_.V - 1] <= 1.0f))
+      fail("genes must lie in [0, 1] (the outer cut-off radius gene exceeds 1)");
+}
+
 void Fitness::compute_async(int P, const float* genes)
 {
   if (P < 1) fail("population size must be positive");
+  check_genes(P, genes);
   PopPlan& pl = plan(P);
   hipStream_t st = stream_;
   std::memcpy(pl.h_genes.get(), genes, sizeof(float) * (size_t)P * spec_.V);
@@ -835,6 +932,7 @@ void Fitness::run_ga(GA& ga, int first_generation, int generations, float* best,
   if (ga.V() != V) fail("the GA's parameter count differs from the potential's");
   if (first_generation < 0) fail("generation numbers start at 0");
   if (generations < 1) return;
+  check_genes(P, ga.population().data());
   PopPlan& pl = plan(P);
   hipStream_t st = stream_;
   // the GA as it was before this call: staging draws advances its random
@@ -969,6 +1067,14 @@ void Fitness::compute(int P, const float* genes, float* fitness, double* terms)
 
 void Fitness::compute_plain(int P, const float* genes, float* fitness, double* terms)
 {
+  plain_loop(P, genes, fitness, terms,
+             PlainBuffers{nullptr, d_f_.get(), d_virial_.get(), d_pe_.get(), d_fit1_.get(),
+                          d_cerr_.get(), d_terms1_.get()});
+}
+
+void Fitness::plain_loop(int P, const float* genes, float* fitness, double* terms,
+                         const PlainBuffers& b)
+{
   // src/fitness.cu:316-333: one parameter set at a time
   std::vector<float> params(spec_.V);
   const FitnessNorm nrm = norm();
@@ -977,14 +1083,13 @@ void Fitness::compute_plain(int P, const float* genes, float* fitness, double* t
     denormalise_host(spec_, genes + (size_t)n * spec_.V, params.data());
     potential_->update_potential(params);
     if (timing_) GPUGA_HIP(hipEventRecord(ev0_, st));
-    potential_->find_force(ds_.view, d_f_.get(), d_virial_.get(), d_pe_.get(), st);
-    launch_fitness_terms(ds_.view, nrm, d_f_.get(), d_virial_.get(), d_pe_.get(), d_cerr_.get(),
-                         d_fit1_.get(), d_terms1_.get(), st);
+    potential_->find_force(ds_.view, b.f, b.virial, b.pe, st, b.ws);
+    launch_fitness_terms(ds_.view, nrm, b.f, b.virial, b.pe, b.cerr, b.fit1, b.terms1, st);
     if (timing_) GPUGA_HIP(hipEventRecord(ev1_, st));
     float fit = 0.0f;
     double t[3];
-    GPUGA_HIP(hipMemcpyAsync(&fit, d_fit1_.get(), sizeof(float), hipMemcpyDeviceToHost, st));
-    GPUGA_HIP(hipMemcpyAsync(t, d_terms1_.get(), sizeof(t), hipMemcpyDeviceToHost, st));
+    GPUGA_HIP(hipMemcpyAsync(&fit, b.fit1, sizeof(float), hipMemcpyDeviceToHost, st));
+    GPUGA_HIP(hipMemcpyAsync(t, b.terms1, sizeof(t), hipMemcpyDeviceToHost, st));
     GPUGA_HIP(hipStreamSynchronize(st));
     if (timing_) {
       float ms = 0.0f;
@@ -1046,11 +1151,13 @@ void Fitness::predict(const std::string& dir, const float* elite)
 
 int Fitness::heuristic_chunk_cost(int P) const
 {
+  const bool tersoff = spec_.type == POT_TERSOFF1 || spec_.type == POT_TERSOFF2;
+  const std::vector<int>& pnn = tersoff ? population_neighbors().NN : nl_.NN;
   long long total = 0, maxcfg = 0;
   for (int c = 0; c < ts_.Nc; ++c) {
     long long t = 0;
     for (int i = ts_.Na_sum[c]; i < ts_.Na_sum[c] + ts_.Na[c]; ++i)
-      t += atom_cost(spec_.type, nl_.NN[i], c < ts_.Nc_force);
+      t += atom_cost(spec_.type, pnn[i], c < ts_.Nc_force);
     total += t;
     maxcfg = std::max(maxcfg, t);
   }
@@ -1100,6 +1207,7 @@ size_t Fitness::estimate_bytes(const PotentialSpec& s, const TrainSet& ts, int P
 
 std::vector<double> Fitness::trace(int P, const float* genes)
 {
+  check_genes(P, genes);
   PopPlan& pl = plan(P);
   hipStream_t st = stream_;
   const size_t n = pl.n_items;
@@ -1160,11 +1268,13 @@ int Fitness::autotune(int P)
   // 1/2 .. 1/32 of the resident waves, and the largest configuration's cost
   // halved step by step (small sets, where latency rather than balance
   // sets the time).
+  const bool tersoff = spec_.type == POT_TERSOFF1 || spec_.type == POT_TERSOFF2;
+  const std::vector<int>& pnn = tersoff ? population_neighbors().NN : nl_.NN;
   long long total = 0, maxcfg = 0;
   for (int c = 0; c < ts_.Nc; ++c) {
     long long t = 0;
     for (int i = ts_.Na_sum[c]; i < ts_.Na_sum[c] + ts_.Na[c]; ++i)
-      t += atom_cost(spec_.type, nl_.NN[i], c < ts_.Nc_force);
+      t += atom_cost(spec_.type, pnn[i], c < ts_.Nc_force);
     total += t;
     maxcfg = std::max(maxcfg, t);
   }

@@ -32,6 +32,10 @@ struct FitnessOptions {
 struct DeviceSet {
   DeviceBuffer<int> Na, Na_sum, type, nn, bond0, nl, rev, ptype, tri0;
   DeviceBuffer<float4> bond, tri, ri_tab, rec, pair_tab;
+  // the Tersoff population kernel's pruned list and its tables, kept when
+  // the looped kernel reads them (more than 16 neighbours within R2)
+  DeviceBuffer<int> p_nn, p_b0, p_nl, p_rev, p_ptype, p_tri0;
+  DeviceBuffer<float4> p_bond, p_tri;
   DeviceBuffer<int4> pair_hdr;
   DeviceBuffer<float> pe_ref, virial_ref, force_ref;
   DevSet view{};
@@ -46,8 +50,11 @@ public:
   virtual ~Potential() = default;
   virtual void initialize(int N, int max_Na) = 0;
   virtual void update_potential(const std::vector<float>& physical) = 0;
+  // ws: device workspace of workspace_floats() floats, or null for the
+  // plug-in's own
   virtual void find_force(const DevSet& set, float* f, float* virial, float* pe,
-                          hipStream_t st) = 0;
+                          hipStream_t st, float* ws = nullptr) = 0;
+  virtual size_t workspace_floats() const { return 0; }
 };
 
 std::unique_ptr<Potential> make_potential(const PotentialSpec& spec, const DeviceSet& ds);
@@ -123,6 +130,10 @@ public:
   const PotentialSpec& spec() const { return spec_; }
   const TrainSet& trainset() const { return ts_; }
   const NeighborList& neighbors() const { return nl_; }
+  // The list the Tersoff population kernel's atom records hold: the bonds
+  // shorter than the largest R2 any individual can have (the whole list
+  // when no bond is that long).
+  const NeighborList& population_neighbors() const { return pruned_ ? pnl_ : nl_; }
   // The device geometry tables (bond, triplet, atom records), copied back.
   void device_geometry(std::vector<float4>& bond, std::vector<float4>& tri,
                        std::vector<float4>& rec) const;
@@ -156,11 +167,24 @@ private:
   [[noreturn]] void discard_launch(PopPlan& pl);
   FitnessNorm norm() const;
   NN2BGrid nn2b_grid_desc() const;
+  // device arrays of the per-individual path
+  struct PlainBuffers {
+    float *ws, *f, *virial, *pe, *fit1;
+    double *cerr, *terms1;
+  };
+  void plain_loop(int P, const float* genes, float* fitness, double* terms,
+                  const PlainBuffers& b);
+  // fails unless genes [P][V] suit the population kernel's tables
+  void check_genes(int P, const float* genes) const;
 
   FitnessOptions opt_;
   PotentialSpec spec_;
   TrainSet ts_;
   NeighborList nl_;
+  NeighborList pnl_;    // pruned list of the Tersoff population kernel
+  bool pruned_ = false; // pnl_ holds fewer bonds than nl_
+  bool loop_path_ = false; // Tersoff: more than 16 neighbours within R2
+  LoopTab loop_{};         // the looped kernel's tables
   Geometry geo_; // host bond geometry of the pair potentials' set-up
   DeviceSet ds_;
   std::unique_ptr<Potential> potential_;

@@ -28,6 +28,18 @@ void launch_fitness_terms(const DevSet& s, const FitnessNorm& nrm, const float* 
 // pack: serve the packed-item queue (several atoms per wave)
 void launch_tersoff_population(const PopCtl& ctl, const DevSet& s, int cls, int grid, int block,
                                bool pack, hipStream_t st);
+// Tersoff sets with more than 16 neighbours within R2: per-bond lane values
+// in LDS (dynamic, tersoff_loop_lds_bytes per workgroup), geometry from the
+// bond and triplet tables of the list in t.
+struct LoopTab {
+  const int *nn, *b0, *tri0, *rev, *ptype; // the list the kernel evaluates
+  const float4 *bond, *tri;                // its bond and triplet tables
+  int max_nn;
+};
+size_t tersoff_loop_lds_bytes(int max_nn, int block);
+void launch_tersoff_loop_population(const PopCtl& ctl, const DevSet& s, const LoopTab& t,
+                                    int grid, int block, hipStream_t st);
+int tersoff_loop_occupancy(bool two, int max_nn, int block);
 void launch_ri_population(const PopCtl& ctl, const DevSet& s, const float4* ri_tab, int grid,
                           int block, hipStream_t st);
 

@@ -156,19 +156,11 @@ __global__ void __launch_bounds__(256) geo_triplets(int N, const int* NN, const 
   }
 }
 
-// Atom records of the Tersoff population kernel, the layout of
-// build_records: header (nn, species-pair mask, first bond, 0), M bonds,
-// M x M triplets, M reverse bonds, M inverse distances, zero-padded.  One
-// thread per float4 of the record array, so that a wave's stores are
-// contiguous.
-__global__ void __launch_bounds__(256) geo_records(int N, int M, const int* NN, const int* start,
-                                                   const int* rev, const int* ptype, const int* tri0,
-                                                   const float4* bond, const float4* tri, float4* rec)
+// One float4 (slot k) of atom i's record.
+__device__ inline void geo_record_entry(int i, int k, int M, const int* NN, const int* start,
+                                        const int* rev, const int* ptype, const int* tri0,
+                                        const float4* bond, const float4* tri, float4* out)
 {
-  const long RS = 1 + M + (long)M * M + M / 2;
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)N * RS) return;
-  const int i = (int)(e / RS), k = (int)(e - (long)i * RS);
   const int nn = NN[i], b0 = start[i];
   const int REV = 1 + M + M * M, INV = REV + M / 4;
   float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -197,7 +189,25 @@ __global__ void __launch_bounds__(256) geo_records(int N, int M, const int* NN, 
     }
     v = make_float4(f[0], f[1], f[2], f[3]);
   }
-  rec[e] = v;
+  *out = v;
+}
+
+// Atom records of the Tersoff population kernel, the layout of
+// build_records: header (nn, species-pair mask, first bond, 0), M bonds,
+// M x M triplets, M reverse bonds, M inverse distances, zero-padded.  One
+// thread per float4 of the record array (grid-stride), so that a wave's
+// stores are contiguous.
+__global__ void __launch_bounds__(256) geo_records(int N, int M, const int* NN, const int* start,
+                                                   const int* rev, const int* ptype, const int* tri0,
+                                                   const float4* bond, const float4* tri, float4* rec)
+{
+  const int RS = 1 + M + M * M + M / 2;
+  const long total = (long)N * RS;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / RS), k = (int)(e - (long)i * RS);
+    geo_record_entry(i, k, M, NN, start, rev, ptype, tri0, bond, tri, rec + e);
+  }
 }
 
 #pragma clang fp contract(on)
@@ -271,8 +281,8 @@ void build_records_device(int N, int M, const DeviceNeighbors& d, const int* pty
   if (N == 0) return;
   if (M != 4 && M != 8 && M != 16) fail("atom records hold 4, 8 or 16 bonds");
   const long total = (long)N * (1 + M + (long)M * M + M / 2);
-  const long blocks = (total + 255) / 256;
-  if (blocks > 0x7fffffffL) fail("atom record table too large");
+  // grid-stride: at most 2^20 workgroups, whatever the table size
+  const long blocks = std::min<long>((total + 255) / 256, 1L << 20);
   geo_records<<<(unsigned)blocks, 256, 0, st>>>(N, M, d.NN, d.start, d.rev, ptype, d.tri0, bond,
                                                 tri, rec);
   GPUGA_HIP(hipGetLastError());

@@ -200,7 +200,7 @@ struct NN2BPop {
     }
   }
 
-  __device__ static double force_pass(const DevSet&, const PopCtl&, int, int, int, int)
+  __device__ static double force_pass(const DevSet&, const PopCtl&, const Extra&, int, int, int, int)
   {
     return 0.0;
   }

@@ -4,25 +4,41 @@
 // the first hidden layer and its r-derivatives (x1, dx1/dr, d2x1/dr2;
 // src/nn2b.cu:52-56 and their derivatives) form three 32 x 32 A operands
 // (grid point x input neuron); the layer-2 weights W1 (src/nn2b.cu:62-66) are
-// the B operand; v_mfma_f32_32x32x8_bf16 produces z, dz/dr, d2z/dr2 for the
-// 32 x 32 (grid point x output neuron) tile.  The epilogue applies tanh and
-// the chain rule per output neuron and folds the output layer
-// (src/nn2b.cu:71-77) through LDS in a fixed order.  n <= 32 (zero padded).
+// the B operand; v_mfma_f32_32x32x16_bf16 produces z, dz/dr, d2z/dr2 for the
+// 32 x 32 (grid point x output neuron) tile.  Every operand is split into two
+// BF16 parts, x = hi + lo with hi = bf16(x) and lo = bf16(x - hi) (16
+// significant bits together), and a product is accumulated as
+// lo_A hi_B + hi_A lo_B + hi_A hi_B (the lo_A lo_B term, 2^-16 relative, is
+// dropped), so the layer keeps nearly FP32 accuracy on BF16 inputs.  A lane
+// holds the same input neurons in its A and B operands, so the products pair
+// up whatever order the instruction gives the K index inside a lane.  The
+// epilogue applies tanh and the chain rule per output neuron and folds the
+// output layer (src/nn2b.cu:71-77) through LDS in a fixed order.  n <= 32
+// (zero padded).
 #include "kernels.h"
 
 namespace gpuga {
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// float -> bfloat16, round to nearest even
-__device__ inline short to_bf16(float f)
+// x = hi + lo, both round-to-nearest-even BF16, into element j of two operands
+__device__ inline void split_bf16(float x, bf16x8& hi, bf16x8& lo, int j)
 {
-  unsigned u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
+  const __bf16 h = (__bf16)x;
+  hi[j] = h;
+  lo[j] = (__bf16)(x - (float)h);
+}
+
+// C += A B with split operands (smallest terms first)
+__device__ inline f32x16 mfma3(const bf16x8& ah, const bf16x8& al, const bf16x8& bh,
+                               const bf16x8& bl, f32x16 c)
+{
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, c, 0, 0, 0);
 }
 
 constexpr int TILE = 32;
@@ -37,18 +53,19 @@ __global__ void __launch_bounds__(64) nn2b_tabulate_bf16(PopCtl ctl, NN2BGrid gr
   const float* gp = ctl.genes + (size_t)p * V;
   auto W = [&](int v) { return denormalise(ctl.pmin[v], ctl.pmax[v], gp[v]); };
 
-  // A-operand neurons of this lane: m = 8 s + 4 kg + j
+  // input neurons of this lane, in its A and B operands alike:
+  // m = 16 s + 8 kg + j (K step s, element j)
   float w0m[16], b0m[16];
-  bf16x4 B[4];
+  bf16x8 Bh[2], Bl[2];
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
+  for (int s = 0; s < 2; ++s) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int m = 8 * s + 4 * kg + j;
+    for (int j = 0; j < 8; ++j) {
+      const int m = 16 * s + 8 * kg + j;
       const bool ok = m < n;
-      w0m[4 * s + j] = ok ? W(m) : 0.0f;
-      b0m[4 * s + j] = ok ? W(n + m) : 0.0f;
-      B[s][j] = to_bf16(ok && col < n ? W(2 * n + col * n + m) : 0.0f);
+      w0m[8 * s + j] = ok ? W(m) : 0.0f;
+      b0m[8 * s + j] = ok ? W(n + m) : 0.0f;
+      split_bf16(ok && col < n ? W(2 * n + col * n + m) : 0.0f, Bh[s], Bl[s], j);
     }
   }
   const float b1c = col < n ? W(n * (n + 2) + col) : 0.0f;
@@ -60,25 +77,22 @@ __global__ void __launch_bounds__(64) nn2b_tabulate_bf16(PopCtl ctl, NN2BGrid gr
     const int g0 = (blockIdx.x * tiles + tt) * TILE;
     if (g0 >= grid.G) break;
     const float d = grid.d0 + grid.h * (float)(g0 + col);
-    bf16x4 AX[4], AY[4], AZ[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const float w = w0m[4 * s + j];
-        const float t = tanhf(w * d - b0m[4 * s + j]);
-        const float sg = 1.0f - t * t;
-        AX[s][j] = to_bf16(t);
-        AY[s][j] = to_bf16(sg * w);
-        AZ[s][j] = to_bf16(-2.0f * t * sg * w * w);
-      }
-    }
     f32x16 Z = {0}, Zd = {0}, Zdd = {0};
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      Z = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(AX[s], B[s], Z, 0, 0, 0);
-      Zd = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(AY[s], B[s], Zd, 0, 0, 0);
-      Zdd = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(AZ[s], B[s], Zdd, 0, 0, 0);
+    for (int s = 0; s < 2; ++s) {
+      bf16x8 AXh, AXl, AYh, AYl, AZh, AZl;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float w = w0m[8 * s + j];
+        const float t = tanhf(w * d - b0m[8 * s + j]);
+        const float sg = 1.0f - t * t;
+        split_bf16(t, AXh, AXl, j);
+        split_bf16(sg * w, AYh, AYl, j);
+        split_bf16(-2.0f * t * sg * w * w, AZh, AZl, j);
+      }
+      Z = mfma3(AXh, AXl, Bh[s], Bl[s], Z);
+      Zd = mfma3(AYh, AYl, Bh[s], Bl[s], Zd);
+      Zdd = mfma3(AZh, AZl, Bh[s], Bl[s], Zdd);
     }
     // this lane holds output neuron `col` for grid rows 8 (r/4) + 4 kg + r%4
 #pragma unroll

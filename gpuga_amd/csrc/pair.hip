@@ -233,7 +233,8 @@ struct RIPop {
     }
   }
 
-  __device__ static double force_pass(const DevSet&, const PopCtl&, int, int, int, int)
+  __device__ static double force_pass(const DevSet&, const PopCtl&, const Extra&, int, int, int,
+                                      int)
   {
     return 0.0;
   }

@@ -300,7 +300,7 @@ __device__ inline double group_sum(double x, int W)
 //   static Lane make(ctl, p);                       denormalised genes of p
 //   static void chunk(set, q, item, ctl, ex, Acc&)  evaluate atoms [a0, a1)
 //   static constexpr bool kForcePass;               three-body force assembly
-//   static double force_pass(set, ctl, a0, a1, k, lane)
+//   static double force_pass(set, ctl, ex, a0, a1, k, lane)
 //                                                   sum |F-F_ref|^2 over [a0, a1)
 //   static constexpr bool kPacked;                  packed items supported:
 //   static void chunk_packed(set, q, item, ctl, ex, col, g, G, Acc&)
@@ -398,7 +398,7 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
             W);
         }
       } else {
-        acc.v[7] = Pot::force_pass(set, ctl, item.a0, item.a1, k, lane);
+        acc.v[7] = Pot::force_pass(set, ctl, ex, item.a0, item.a1, k, lane);
       }
       unsigned* cnt2 = &ctl.cnt2[(long)c * ctl.K + k];
       if (nch > 1) {

@@ -452,8 +452,8 @@ struct TersoffPop {
   // from the per-bond forces every chunk of the configuration published, then
   // sum_i |F_i - F_ref,i|^2 (src/minimal_tersoff.cu:355-383,
   // src/fitness.cu:392-435).
-  __device__ static double force_pass(const DevSet& s, const PopCtl& ctl, int a0, int a1, int k,
-                                      int lane)
+  __device__ static double force_pass(const DevSet& s, const PopCtl& ctl, const Extra&, int a0,
+                                      int a1, int k, int lane)
   {
     // per lane block: bond_force slots of received f_ji, then one slot per
     // force-configuration atom with its own sum_j f_ij (written in phase 0)
@@ -523,6 +523,181 @@ struct TersoffPop {
     return err;
   }
 };
+
+// Sets with more than 16 neighbours within R2: the reference's bond loops
+// (src/minimal_tersoff.cu:122-316) with run-time trip counts, the geometry
+// read from the bond and triplet tables (wave-uniform loads) and the per-bond
+// values of the 64 lanes (fc, fc'/d, b' fA, the pair part of f_ij) kept in
+// LDS rows, lane-contiguous, one region per wave.
+template <bool TWO>
+struct TersoffLoop {
+  static constexpr bool kForcePass = true;
+  static constexpr bool kPacked = false;
+  using Lane = TLane;
+  using Extra = LoopTab;
+
+  __device__ static Lane make(const PopCtl& ctl, int p) { return TersoffPop<4, TWO>::make(ctl, p); }
+
+  template <bool FORCE>
+  __device__ static void atom(const DevSet& s, const LoopTab& t, const TLane& q, int i,
+                              float* fblk, int lane, float* L, Acc& acc)
+  {
+    const int nn = ldu(t.nn, i), b0 = ldu(t.b0, i), t0 = ldu(t.tri0, i);
+    const int M = t.max_nn;
+    float* FC = L + lane;                    // fc(d_v)
+    float* FCPD = L + M * WAVE + lane;       // fc'(d_v) / d_v
+    float* AB = L + 2 * M * WAVE + lane;     // b'_v fA_v
+    float* PF = L + 3 * M * WAVE + lane;     // pair part of f_ij, over d
+    for (int v = 0; v < nn; ++v) {
+      const float d = ldu(t.bond, b0 + v).w;
+      float fc, fcp;
+      tersoff_fc(q, d, fc, fcp);
+      FC[v * WAVE] = fc;
+      FCPD[v * WAVE] = fcp * frcp(d);
+    }
+    float e = 0.0f;
+    for (int u = 0; u < nn; ++u) {
+      float zeta = 0.0f;
+      for (int v = 0; v < nn; ++v) {
+        if (v == u) continue;
+        const float x = ldu(t.tri, t0 + u * nn + v).x - q.h;
+        zeta = fmaf(FC[v * WAVE] * q.beta, x * x, zeta);
+      }
+      float b, bp;
+      tersoff_bond_order(q, zeta, b, bp);
+      const float d = ldu(t.bond, b0 + u).w;
+      float fr, frp, fa, fap;
+      if (TWO && (ldu(t.ptype, b0 + u) & 1)) tersoff_pair(q, 1, d, fr, frp, fa, fap);
+      else tersoff_pair(q, 0, d, fr, frp, fa, fap);
+      const float fc = FC[u * WAVE];
+      const float rep = fr - b * fa;
+      e = fmaf(fc, rep, e);
+      // (fc' rep + fc (fR' - b fA')) / d, with fc' / d from the first loop
+      PF[u * WAVE] = fmaf(FCPD[u * WAVE], rep, fc * (frp - b * fap) * frcp(d));
+      AB[u * WAVE] = bp * fa;
+    }
+    float wxx = 0, wyy = 0, wzz = 0, wxy = 0, wyz = 0, wzx = 0;
+    for (int u = 0; u < nn; ++u) {
+      const float Au = AB[u * WAVE];
+      float cx = 0.0f, cy = 0.0f, cz = 0.0f, dsum = 0.0f;
+      for (int v = 0; v < nn; ++v) {
+        if (v == u) continue;
+        const float4 tr = ldu(t.tri, t0 + u * nn + v);
+        const float x = tr.x - q.h;
+        const float fcv = FC[v * WAVE];
+        const float Bv = AB[v * WAVE] * fcv;
+        const float w = fmaf(Au, fcv, Bv) * x;
+        cx = fmaf(w, tr.y, cx);
+        cy = fmaf(w, tr.z, cy);
+        cz = fmaf(w, tr.w, cz);
+        dsum = fmaf(Bv * x, x, dsum);
+      }
+      const float4 r = ldu(t.bond, b0 + u);
+      const float coef = 0.5f * (PF[u * WAVE] - FCPD[u * WAVE] * (q.beta * dsum));
+      const float hc = FC[u * WAVE] * q.beta;
+      const float fx = coef * r.x - hc * cx;
+      const float fy = coef * r.y - hc * cy;
+      const float fz = coef * r.z - hc * cz;
+      if (FORCE) {
+        const int rb = ldu(t.rev, b0 + u);
+        float* o = fblk + (size_t)rb * 3 * WAVE + lane;
+        cstore(o, fx);
+        cstore(o + WAVE, fy);
+        cstore(o + 2 * WAVE, fz);
+        wxx += fx;
+        wyy += fy;
+        wzz += fz;
+      } else {
+        wxx -= r.x * fx;
+        wyy -= r.y * fy;
+        wzz -= r.z * fz;
+        wxy -= r.x * fy;
+        wyz -= r.y * fz;
+        wzx -= r.z * fx;
+      }
+    }
+    if (FORCE) {
+      float* o = fblk + ((size_t)s.bond_force + i) * 3 * WAVE + lane;
+      cstore(o, wxx);
+      cstore(o + WAVE, wyy);
+      cstore(o + 2 * WAVE, wzz);
+    } else {
+      acc.v[0] += (double)(0.5f * e);
+      acc.v[1] += (double)wxx;
+      acc.v[2] += (double)wyy;
+      acc.v[3] += (double)wzz;
+      acc.v[4] += (double)wxy;
+      acc.v[5] += (double)wyz;
+      acc.v[6] += (double)wzx;
+    }
+  }
+
+  __device__ static void chunk(const DevSet& s, const TLane& q, const WorkItem& it,
+                               const PopCtl& ctl, const LoopTab& t, Acc& acc)
+  {
+    extern __shared__ float loop_lds[];
+    const int lane = __lane_id();
+    float* L = loop_lds + (size_t)(threadIdx.x / WAVE) * 4 * t.max_nn * WAVE;
+    if (it.c < s.Nc_force) {
+      float* fblk = ctl.bondf + (size_t)it.k * ctl.B_force * 3 * WAVE;
+      for (int i = it.a0; i < it.a1; ++i) atom<true>(s, t, q, i, fblk, lane, L, acc);
+    } else {
+      for (int i = it.a0; i < it.a1; ++i) atom<false>(s, t, q, i, nullptr, lane, L, acc);
+    }
+  }
+
+  // F_i = sum_j (f_ij - f_ji) over the list the phase-0 items used, then
+  // sum_i |F_i - F_ref,i|^2 (src/minimal_tersoff.cu:355-383)
+  __device__ static double force_pass(const DevSet& s, const PopCtl& ctl, const LoopTab& t,
+                                      int a0, int a1, int k, int lane)
+  {
+    const float* fblk = ctl.bondf + (size_t)k * ctl.B_force * 3 * WAVE + lane;
+    const long N = s.N;
+    double err = 0.0;
+    for (int i = a0; i < a1; ++i) {
+      const int nn = ldu(t.nn, i), b0 = ldu(t.b0, i);
+      const float* own = fblk + ((size_t)s.bond_force + i) * 3 * WAVE;
+      float fx = cload(own), fy = cload(own + WAVE), fz = cload(own + 2 * WAVE);
+      for (int u = 0; u < nn; ++u) {
+        const float* r = fblk + (size_t)(b0 + u) * 3 * WAVE;
+        fx -= cload(r);
+        fy -= cload(r + WAVE);
+        fz -= cload(r + 2 * WAVE);
+      }
+      const float dx = fx - ldu(s.force_ref, i), dy = fy - ldu(s.force_ref, i + N),
+                  dz = fz - ldu(s.force_ref, i + 2 * N);
+      err += (double)dx * dx + (double)dy * dy + (double)dz * dz;
+    }
+    return err;
+  }
+};
+
+size_t tersoff_loop_lds_bytes(int max_nn, int block)
+{
+  return (size_t)(block / WAVE) * 4 * max_nn * WAVE * sizeof(float);
+}
+
+void launch_tersoff_loop_population(const PopCtl& ctl, const DevSet& s, const LoopTab& t,
+                                    int grid, int block, hipStream_t st)
+{
+  const size_t lds = tersoff_loop_lds_bytes(t.max_nn, block);
+  if (ctl.V == 13)
+    population_kernel<TersoffLoop<true>, false><<<grid, block, lds, st>>>(ctl, s, t);
+  else
+    population_kernel<TersoffLoop<false>, false><<<grid, block, lds, st>>>(ctl, s, t);
+  GPUGA_HIP(hipGetLastError());
+}
+
+int tersoff_loop_occupancy(bool two, int max_nn, int block)
+{
+  int n = 0;
+  const void* f =
+    two ? reinterpret_cast<const void*>(&population_kernel<TersoffLoop<true>, false>)
+        : reinterpret_cast<const void*>(&population_kernel<TersoffLoop<false>, false>);
+  GPUGA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block,
+                                                         tersoff_loop_lds_bytes(max_nn, block)));
+  return n;
+}
 
 template <int NMAX, bool TWO>
 static void launch_t(const PopCtl& ctl, const DevSet& s, int grid, int block, bool pack,

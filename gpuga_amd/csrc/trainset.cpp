@@ -4,6 +4,7 @@
 #include "common.h"
 
 #include <algorithm>
+#include <limits>
 #include <cctype>
 #include <cerrno>
 #include <cmath>
@@ -518,6 +519,45 @@ std::vector<int> bond_types(const TrainSet& ts, const NeighborList& nl, bool thr
 int record_slots(int max_nn)
 {
   return max_nn <= 4 ? 4 : max_nn <= 8 ? 8 : max_nn <= 16 ? 16 : 0;
+}
+
+float tersoff_reach(const PotentialSpec& s)
+{
+  if ((s.type != POT_TERSOFF1 && s.type != POT_TERSOFF2) || s.V < 1)
+    return std::numeric_limits<float>::infinity();
+  // the kernels' denormalisation lo + (hi - lo) * g, unfused, at g = 1
+  volatile float lo = s.pmin[s.V - 1], hi = s.pmax[s.V - 1];
+  volatile float b = hi - lo;
+  volatile float bx = b * 1.0f;
+  return lo + bx;
+}
+
+NeighborList prune_neighbors(const NeighborList& nl, const std::vector<float>& d, float reach)
+{
+  const int N = (int)nl.NN.size();
+  NeighborList p;
+  p.NN.assign(N, 0);
+  p.start.assign(N, 0);
+  std::vector<int> map(nl.total, -1); // full bond index -> pruned index
+  long long total = 0;
+  for (int i = 0; i < N; ++i) {
+    p.start[i] = (int)total;
+    for (int b = nl.start[i]; b < nl.start[i] + nl.NN[i]; ++b)
+      if (d[b] < reach) map[b] = (int)total++;
+    p.NN[i] = (int)total - p.start[i];
+    p.max_nn = std::max(p.max_nn, p.NN[i]);
+  }
+  p.total = (int)total;
+  p.NL.resize(total);
+  p.rev.resize(total);
+  for (int b = 0; b < nl.total; ++b) {
+    if (map[b] < 0) continue;
+    const int r = map[nl.rev[b]];
+    if (r < 0) fail("internal: pruned neighbour list is not symmetric");
+    p.NL[map[b]] = nl.NL[b];
+    p.rev[map[b]] = r;
+  }
+  return p;
 }
 
 std::vector<Vec4> build_records(const NeighborList& nl, const Geometry& g,

@@ -90,6 +90,18 @@ std::vector<int> bond_types(const TrainSet& ts, const NeighborList& nl, bool thr
 // first bond, 0), M bonds, M x M triplets (s, t), M reverse-bond ints and M
 // inverse distances, zero-padded.
 int record_slots(int max_nn);
+// Largest outer cut-off radius R2 any individual of a minimal-Tersoff
+// potential can have: the last parameter's pmin + (pmax - pmin) g at g = 1,
+// in the float arithmetic of the denormalisation (monotonic in g, so every
+// g in [0, 1] gives at most this); +inf for other potentials.
+float tersoff_reach(const PotentialSpec& s);
+// The bonds of nl with |r_ij| < reach (d[b] = |r_b| in FP32, as the
+// geometry table holds it).  A bond at or beyond every individual's R2 has
+// fc = fc' = 0 (src/minimal_tersoff.cu:77-90), so every term it enters is
+// an exact zero: energy, forces and virial are unchanged without it.  A
+// bond's reverse has the same length (exact minimum image), so the pruned
+// list is symmetric; reverse indices are renumbered.
+NeighborList prune_neighbors(const NeighborList& nl, const std::vector<float>& d, float reach);
 std::vector<Vec4> build_records(const NeighborList& nl, const Geometry& g,
                                 const std::vector<int>& ptype, int M);
 

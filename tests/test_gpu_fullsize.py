@@ -105,13 +105,13 @@ def test_full_size_rigid_ion_matches_oracle():
     np.testing.assert_allclose(f[sel], ref, rtol=1e-5)
 
 
-@pytest.mark.parametrize("bf16,tol", [(0, 1e-5), (1, 3e-2)])
+@pytest.mark.parametrize("bf16,tol", [(0, 1e-5), (1, 1e-4)])
 def test_full_size_nn2b_matches_oracle(bf16, tol):
     """The NN2B wurtzite workload at full size (800 x 72 atoms, n = 32,
     1153 parameters, P = 256): one individual of every lane block against the
     oracle, fitness and each of the three terms (force included); FP32
-    tabulation within 1e-5 relative, the BF16 hidden layer within its stated
-    3e-2."""
+    tabulation within 1e-5 relative, the split-BF16 hidden layer within its
+    stated 1e-4."""
     import gpuga_amd
     from gpuga_amd import synth
     print_case("full size nn2b vs oracle", bf16)

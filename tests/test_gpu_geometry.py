@@ -42,12 +42,18 @@ def test_golden_geometry(golden, name):
         assert len(rec) == 0
 
 
-@pytest.mark.parametrize("cutoff,slots", [(3.0, 4), (3.5, 8), (3.9, 16)])
-def test_synthetic_tersoff_geometry(tmp_path, cutoff, slots):
+@pytest.mark.parametrize("cutoff,r2max,slots", [(3.0, 3.2, 4), (3.5, 3.5, 8), (3.9, 3.9, 16),
+                                                (3.9, 3.2, 4)])
+def test_synthetic_tersoff_geometry(tmp_path, cutoff, r2max, slots):
+    """Records of 4, 8 and 16 bond slots; with R2 at most 3.2 A under a 3.9 A
+    cut-off, the records hold only the bonds shorter than 3.2 A (the longer
+    ones contribute exact zeros), on the device as on the host."""
     from gpuga_amd import synth
-    print_case("geometry synthetic", cutoff)
+    print_case("geometry synthetic", cutoff, r2max)
     d = synth.tersoff_si(Nc=12, reps=(3, 3, 3), seed=4)
     d["cutoff"] = cutoff
+    d["pmax"] = d["pmax"].copy()
+    d["pmax"][-1] = r2max
     synth.write_dir(d, str(tmp_path))
     bond, tri, rec = _check(str(tmp_path))
     N = int(np.sum(d["Na"]))

@@ -195,3 +195,24 @@ def test_host_geometry_tables(golden, name):
         assert R[0].view(np.int32)[0] == nn and R[0].view(np.int32)[2] == b0
         assert np.array_equal(R[1:1 + nn], bond[b0:b0 + nn].astype(np.float32))
         assert np.array_equal(R[1 + M:1 + M + M * M].reshape(M, M, 4)[:nn, :nn], T)
+
+
+def test_pruned_records_equal_records_of_the_shorter_list(tmp_path):
+    """A 4.6 A cut-off (28 neighbours) with R2 at most 3.2 A: the population
+    kernel's atom records hold only the bonds shorter than 3.2 A, and are
+    then bit for bit
+    the records of the same set under a 3.2 A cut-off (the same bonds in
+    the same order, with their reverse-bond and first-bond indices)."""
+    from gpuga_amd import synth
+    d = synth.tersoff_si(Nc=6, reps=(3, 3, 3), seed=9)
+    recs = {}
+    for cutoff in (4.6, 3.2):
+        d["cutoff"] = cutoff
+        path = str(tmp_path / str(cutoff))
+        synth.write_dir(d, path)
+        NN, _, _ = G.inspect_neighbors(path)
+        recs[cutoff] = (int(NN.max()), G.inspect_geometry(path)[2])
+    assert recs[4.6][0] == 28 and recs[3.2][0] == 4
+    a, b = recs[4.6][1], recs[3.2][1]
+    assert len(a) == int(np.sum(d["Na"])) * (1 + 4 + 16 + 2)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
