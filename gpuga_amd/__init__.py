@@ -76,6 +76,7 @@ def lib():
             "gpuga_fitness_compute": (C.c_int, [vp, C.c_int, fp, fp]),
             "gpuga_fitness_compute_terms": (C.c_int, [vp, C.c_int, fp, fp, dp]),
             "gpuga_fitness_compute_plain": (C.c_int, [vp, C.c_int, fp, fp, dp]),
+            "gpuga_fitness_compute_plain_noalloc": (C.c_int, [vp, C.c_int, fp, fp, dp]),
             "gpuga_fitness_predict": (C.c_int, [vp, C.c_char_p, fp]),
             "gpuga_fitness_find_force": (C.c_int, [vp, fp, fp, fp, fp]),
             "gpuga_fitness_neighbors": (C.c_int, [vp, ip, ip, ip]),

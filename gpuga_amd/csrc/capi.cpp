@@ -232,6 +232,17 @@ int gpuga_fitness_compute_plain(gpuga_fitness* h, int P, const float* genes, flo
   });
 }
 
+int gpuga_fitness_compute_plain_noalloc(gpuga_fitness* h, int P, const float* genes,
+                                        float* fitness, double* terms)
+{
+  return guarded([&] {
+    need(h, "handle");
+    need(genes, "genes");
+    need(fitness, "fitness");
+    h->f->compute_plain_noalloc(P, genes, fitness, terms);
+  });
+}
+
 int gpuga_fitness_predict(gpuga_fitness* h, const char* dir, const float* elite)
 {
   return guarded([&] {

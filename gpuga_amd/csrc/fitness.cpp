@@ -1,5 +1,6 @@
 // Fitness evaluator host code: set-up, plug-ins, population plans, predict.
 #include "fitness.h"
+#include "noalloc.h"
 
 #include <algorithm>
 #include <cmath>
@@ -465,6 +466,9 @@ void Fitness::setup()
 
   potential_ = make_potential(spec_, ds_);
   potential_->initialize(ts_.N, ts_.max_Na);
+  // this device's fixed workspace of the allocation-free path, looked up
+  // here (set-up) so that the path's calls only use the kept address
+  ws_static_ = static_workspace();
   d_pmin_.upload(spec_.pmin, st);
   d_pmax_.upload(spec_.pmax, st);
   GPUGA_HIP(hipStreamSynchronize(st));
@@ -1070,6 +1074,47 @@ void Fitness::compute_plain(int P, const float* genes, float* fitness, double* t
   plain_loop(P, genes, fitness, terms,
              PlainBuffers{nullptr, d_f_.get(), d_virial_.get(), d_pe_.get(), d_fit1_.get(),
                           d_cerr_.get(), d_terms1_.get()});
+}
+
+size_t Fitness::carve_plain(char* base, PlainBuffers* b) const
+{
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    off = (off + 255) / 256 * 256;
+    char* p = base ? base + off : nullptr;
+    off += bytes;
+    return p;
+  };
+  const size_t N = ts_.N, Nc = ts_.Nc;
+  PlainBuffers x;
+  const size_t ws = potential_->workspace_floats();
+  x.ws = ws ? reinterpret_cast<float*>(take(ws * sizeof(float))) : nullptr;
+  x.f = reinterpret_cast<float*>(take(3 * N * sizeof(float)));
+  x.virial = reinterpret_cast<float*>(take(6 * N * sizeof(float)));
+  x.pe = reinterpret_cast<float*>(take(N * sizeof(float)));
+  x.fit1 = reinterpret_cast<float*>(take(sizeof(float)));
+  x.cerr = reinterpret_cast<double*>(take(3 * Nc * sizeof(double)));
+  x.terms1 = reinterpret_cast<double*>(take(3 * sizeof(double)));
+  if (b) *b = x;
+  return off;
+}
+
+size_t Fitness::noalloc_bytes() const { return carve_plain(nullptr, nullptr); }
+
+void Fitness::compute_plain_noalloc(int P, const float* genes, float* fitness, double* terms)
+{
+  const size_t need = noalloc_bytes();
+  if (need > kStaticWorkspaceBytes)
+    fail(strfmt("the allocation-free path needs %zu bytes of device workspace for this "
+                "training set; the fixed workspace holds %zu",
+                need, kStaticWorkspaceBytes));
+  if (!ws_static_) fail("the fixed device workspace was not set up on this device");
+  int dev = 0;
+  GPUGA_HIP(hipGetDevice(&dev));
+  if (dev != device_) fail("the handle's device is not the current device");
+  PlainBuffers b;
+  carve_plain(static_cast<char*>(ws_static_), &b);
+  plain_loop(P, genes, fitness, terms, b);
 }
 
 void Fitness::plain_loop(int P, const float* genes, float* fitness, double* terms,

@@ -98,6 +98,12 @@ public:
   void compute(int P, const float* genes, float* fitness, double* terms = nullptr);
   // The per-individual-launch form (one find_force + reduction + sync each).
   void compute_plain(int P, const float* genes, float* fitness, double* terms = nullptr);
+  // The same with every device scratch array in the fixed per-GPU workspace
+  // of noalloc.h: no device memory is allocated or freed.  Fails when the
+  // training set needs more than that workspace holds.
+  void compute_plain_noalloc(int P, const float* genes, float* fitness, double* terms = nullptr);
+  // Workspace bytes compute_plain_noalloc needs for this training set.
+  size_t noalloc_bytes() const;
   // Fitness::predict (src/fitness.cu:347-390): writes force/energy/virial.out.
   void predict(const std::string& dir, const float* elite);
   // Per-atom outputs of one physical parameter set (host arrays).
@@ -174,6 +180,9 @@ private:
   };
   void plain_loop(int P, const float* genes, float* fitness, double* terms,
                   const PlainBuffers& b);
+  // the per-individual path's arrays laid out from base (256-byte aligned);
+  // returns the bytes they span (base may be null to measure)
+  size_t carve_plain(char* base, PlainBuffers* b) const;
   // fails unless genes [P][V] suit the population kernel's tables
   void check_genes(int P, const float* genes) const;
 
@@ -198,6 +207,7 @@ private:
   // per-atom outputs and plain-path scratch
   DeviceBuffer<float> d_f_, d_virial_, d_pe_, d_params_, d_fit1_, d_pmin_, d_pmax_;
   DeviceBuffer<double> d_cerr_, d_terms1_;
+  void* ws_static_ = nullptr; // this device's fixed workspace (noalloc.h)
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   bool timing_ = false;
   int spin_limit_ = kSpinLimit;

@@ -82,6 +82,19 @@ int gpuga_fitness_compute_terms(gpuga_fitness* h, int P, const float* genes, flo
 int gpuga_fitness_compute_plain(gpuga_fitness* h, int P, const float* genes, float* fitness,
                                 double* terms);
 
+/* gpuga_fitness_compute_plain with every device scratch array (the
+ * per-bond and per-atom arrays of find_force, the per-configuration errors)
+ * in a fixed workspace of 64 MiB per GPU, a file-scope device array set up
+ * with the handle: the call allocates and frees no device memory.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops).  Fails
+ * when the training set needs more than 64 MiB of workspace (5 floats per
+ * bond for minimal Tersoff, 10 floats per atom, 3 doubles per
+ * configuration).  All calls on one GPU share its workspace: calls that
+ * overlap on one GPU (two threads, two handles) race. */
+int gpuga_fitness_compute_plain_noalloc(gpuga_fitness* h, int P, const float* genes,
+                                        float* fitness, double* terms);
+
 /* Fitness::predict(char* dir, float* elite), src/fitness.cu:347-390: writes
  * force.out, energy.out, virial.out into dir. */
 int gpuga_fitness_predict(gpuga_fitness* h, const char* dir, const float* elite_genes);
