@@ -28,10 +28,10 @@ struct DevSet {
   const float4* tri;      // [T] cos theta, d cos / d r_ij (Tersoff only)
   // Tersoff atom records, one per atom at a fixed stride of 1 + M + M*M +
   // M/2 float4 (M = neighbour bound of the set): header {nn, unlike-type bit
-  // mask, first bond, 0}, then M bonds (x, y, z, d), then the M x M
-  // triplets (cos theta_jik, d cos / d r_ij), then M reverse-bond indices
-  // (int) and M inverse distances 1/d.  Every load of an atom is independent
-  // of the others, so they issue together.
+  // mask, first bond, longest bond's length}, then M bonds (x, y, z, d),
+  // then the M x M triplets (cos theta_jik, d cos / d r_ij), then M
+  // reverse-bond indices (int) and M inverse distances 1/d.  Every load of
+  // an atom is independent of the others, so they issue together.
   const float4* rec;
   int rec_nmax;
   int bond_force;         // bonds of the force configurations (they come first)

@@ -679,8 +679,9 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   // largest chunk first, the lane blocks of a chunk together; a force-assembly
   // item follows the last phase-0 item of its configuration by about one
   // item per resident wave of the queue.
-  const int nq = std::max(1, std::min(opt_.queues > 0 ? opt_.queues : kQueues,
-                                      pl.grid * waves_per_block));
+  // (at most 64: the kernel keeps one bit per queue in its closed mask)
+  const int nq = std::max(1, std::min({opt_.queues > 0 ? opt_.queues : kQueues,
+                                       pl.grid * waves_per_block, 64}));
   std::vector<long long> cfg_cost(Nc, 0);
   for (const Costed& e : eval) cfg_cost[e.w.c] += e.cost;
   for (const Costed& e : assemble) cfg_cost[e.w.c] += e.cost;
@@ -752,7 +753,10 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   // counters: cnt [Nc*K] | cnt2 [Nc*K] | grp_done [K*groups] | blk_done [K] |
   // (pad to 4 KiB) fetch [nq, kFetchStride apart] | done
   const size_t fetch0 = ((size_t)2 * Nc * K + (size_t)K * groups + K + 1023) / 1024 * 1024;
-  pl.counters.resize(fetch0 + (size_t)nq * kFetchStride + 1);
+  // ... then done (one word) and, 8-byte aligned in the next 256 bytes, the
+  // closed-queue mask
+  const size_t closed0 = fetch0 + (size_t)nq * kFetchStride + 64;
+  pl.counters.resize(closed0 + 2);
   pl.counters.zero(st);
   pl.h_genes.resize((size_t)P * spec_.V);
   pl.h_out.resize(out_doubles);
@@ -782,6 +786,7 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   c.blk_done = c.grp_done + (size_t)K * groups;
   c.fetch = ctr + fetch0;
   c.done = c.fetch + (size_t)nq * kFetchStride;
+  c.closed = reinterpret_cast<unsigned long long*>(ctr + closed0);
   c.terms = pl.out.get();
   c.fitness = reinterpret_cast<float*>(c.terms + 3 * (size_t)P);
   c.status = reinterpret_cast<unsigned*>(c.fitness + P);

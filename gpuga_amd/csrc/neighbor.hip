@@ -166,8 +166,12 @@ __device__ inline void geo_record_entry(int i, int k, int M, const int* NN, cons
   float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (k == 0) {
     int mask = 0;
-    for (int u = 0; u < nn; ++u) mask |= (ptype[b0 + u] & 1) << u;
-    v = make_float4(__int_as_float(nn), __int_as_float(mask), __int_as_float(b0), 0.0f);
+    float dmax = 0.0f;
+    for (int u = 0; u < nn; ++u) {
+      mask |= (ptype[b0 + u] & 1) << u;
+      dmax = fmaxf(dmax, bond[b0 + u].w);
+    }
+    v = make_float4(__int_as_float(nn), __int_as_float(mask), __int_as_float(b0), dmax);
   } else if (k <= M) {
     if (k - 1 < nn) v = bond[b0 + k - 1];
   } else if (k < REV) {
@@ -193,10 +197,10 @@ __device__ inline void geo_record_entry(int i, int k, int M, const int* NN, cons
 }
 
 // Atom records of the Tersoff population kernel, the layout of
-// build_records: header (nn, species-pair mask, first bond, 0), M bonds,
-// M x M triplets, M reverse bonds, M inverse distances, zero-padded.  One
-// thread per float4 of the record array (grid-stride), so that a wave's
-// stores are contiguous.
+// build_records: header (nn, species-pair mask, first bond, longest bond),
+// M bonds, M x M triplets, M reverse bonds, M inverse distances,
+// zero-padded.  One thread per float4 of the record array (grid-stride), so
+// that a wave's stores are contiguous.
 __global__ void __launch_bounds__(256) geo_records(int N, int M, const int* NN, const int* start,
                                                    const int* rev, const int* ptype, const int* tri0,
                                                    const float4* bond, const float4* tri, float4* rec)

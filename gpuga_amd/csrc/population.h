@@ -92,6 +92,7 @@ struct PopCtl {
   unsigned* blk_done;     // [K]
   unsigned* fetch;        // [nq] items handed out per queue, kFetchStride apart
   unsigned* done;         // [1] waves finished
+  unsigned long long* closed; // bit q: every item of queue q has been handed out
   unsigned* status;       // [1] non-zero: a dependency wait timed out
   int spin_limit;         // polls of a dependency wait before it gives up
                           // (about 1 us each); < 0: give up at once
@@ -159,19 +160,29 @@ __device__ inline bool wave_wait_count(const unsigned* p, unsigned target, int l
   }
 }
 
-// Sum n rows (stride `stride` doubles) of three arrays, 4 independent partial
-// sums per array so the loads overlap; fixed order.
+// Sum n rows (stride `stride` doubles) of three arrays in a fixed order.  The
+// rows are read from memory (coherent loads), so the time is the number of
+// round trips: the loads of 8 rows (24 values) are all issued before any is
+// added, then the next 8, with 4 partial sums per array.
 __device__ inline void sum3(const double* e0, const double* e1, const double* e2, long stride,
                             int n, double out[3])
 {
+  constexpr int B = 8;
   double s0[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
   int c = 0;
-  for (; c + 4 <= n; c += 4) {
+  for (; c + B <= n; c += B) {
+    double v0[B], v1[B], v2[B];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      s0[u] += cload(e0 + (c + u) * stride);
-      s1[u] += cload(e1 + (c + u) * stride);
-      s2[u] += cload(e2 + (c + u) * stride);
+    for (int u = 0; u < B; ++u) {
+      v0[u] = cload(e0 + (c + u) * stride);
+      v1[u] = cload(e1 + (c + u) * stride);
+      v2[u] = cload(e2 + (c + u) * stride);
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      s0[u & 3] += v0[u];
+      s1[u & 3] += v1[u];
+      s2[u & 3] += v2[u];
     }
   }
   for (; c < n; ++c) {
@@ -245,7 +256,21 @@ __device__ inline bool fold_chunks(const PopCtl& ctl, unsigned* counter, int c, 
   observe();
 #pragma unroll
   for (int u = 0; u < 8; ++u) acc.v[u] = 0.0;
-  for (int ch = 0; ch < nch; ++ch) {
+  // two chunks' partials (16 values) loaded per round trip, added in chunk order
+  int ch = 0;
+  for (; ch + 2 <= nch; ch += 2) {
+    const double* s = ctl.part + ((slot0 + ch) * ctl.K + k) * 8 * WAVE;
+    const double* t = s + (long)ctl.K * 8 * WAVE;
+    double a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a[u] = cload(s + u * WAVE + lane);
+      b[u] = cload(t + u * WAVE + lane);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc.v[u] = (acc.v[u] + a[u]) + b[u];
+  }
+  if (ch < nch) {
     const double* s = ctl.part + ((slot0 + ch) * ctl.K + k) * 8 * WAVE;
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc.v[u] += cload(s + u * WAVE + lane);
@@ -263,28 +288,34 @@ __device__ __forceinline__ unsigned xcc_id()
 }
 
 // Next item for this wave: from queue q while it has items, then from the
-// following queues; -1 when every queue is empty.  The current queue is
-// fetched from directly (one atomic); another queue's counter is read first,
-// so an empty queue costs a load, not an atomic.  Counters are kFetchStride
-// words apart, each in its own memory channel's lines.
+// next queue (cyclically) that is still open; -1 once every queue is
+// closed.  The fetch that first runs past the end of a queue marks it closed
+// (one atomic OR per queue and launch); the others read the closed mask when
+// their fetch comes back empty, so a wave finds an open queue, or learns that
+// all are closed, with one load instead of trying the queues one by one.
+// Counters are kFetchStride words apart, each in its own memory channel's
+// lines.
 __device__ inline int next_item(const PopCtl& ctl, int& q)
 {
-  for (int tries = 0; tries < ctl.nq; ++tries) {
+  const unsigned long long all = ctl.nq >= 64 ? ~0ull : (1ull << ctl.nq) - 1ull;
+  for (;;) {
     unsigned* f = ctl.fetch + (long)q * kFetchStride;
     const unsigned q0 = (unsigned)ldu(ctl.qstart, q), len = (unsigned)ldu(ctl.qstart, q + 1) - q0;
-    bool open = tries == 0;
-    if (!open) {
-      unsigned seen = 0;
-      if (__lane_id() == 0) seen = cload(f);
-      open = (unsigned)wave_first((int)seen) < len;
+    const unsigned it = wave_atomic_add(f, 1u);
+    if (it < len) return (int)(q0 + it);
+    unsigned long long closed = 0;
+    if (__lane_id() == 0) {
+      if (it == len) closed = atomicOr(ctl.closed, 1ull << q) | (1ull << q);
+      else closed = cload(ctl.closed) | (1ull << q);
     }
-    if (open) {
-      const unsigned it = wave_atomic_add(f, 1u);
-      if (it < len) return (int)(q0 + it);
-    }
-    q = q + 1 == ctl.nq ? 0 : q + 1;
+    closed = ((unsigned long long)(unsigned)wave_first((int)(unsigned)(closed >> 32)) << 32) |
+             (unsigned)wave_first((int)(unsigned)closed);
+    const unsigned long long open = ~closed & all;
+    if (open == 0) return -1;
+    // the first open queue after q, else the first open one
+    const unsigned long long above = q >= 63 ? 0ull : open & (~0ull << (q + 1));
+    q = __builtin_ctzll(above ? above : open);
   }
-  return -1;
 }
 
 // Sum over the G lane groups of a packed item: lanes l, l ^ W, l ^ 2W, ...
@@ -344,6 +375,7 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
       const unsigned d = wave_atomic_add(ctl.done, 1u);
       if (d == total_waves - 1u && lane == 0) {
         for (int j = 0; j < ctl.nq; ++j) cstore(ctl.fetch + (long)j * kFetchStride, 0u);
+        cstore(ctl.closed, 0ull);
         cstore(ctl.done, 0u);
       }
       break;

@@ -24,6 +24,7 @@ namespace gpuga {
 struct TLane {
   float cR[2], kR[2], kR2[2], cA[2], kA[2], kA2[2], r0[2];
   float en, mhon, en1, mhon1, beta, beta2, h, r1, r2, inv_span, pif_half;
+  float r1min; // population kernels: the smallest R1 of the wave's lanes
 };
 
 __host__ __device__ inline TLane tersoff_lane(const TersoffParams& p)
@@ -49,6 +50,7 @@ __host__ __device__ inline TLane tersoff_lane(const TersoffParams& p)
   q.r2 = p.r2;
   q.inv_span = 1.0f / (p.r2 - p.r1);
   q.pif_half = 0.5f * p.pi_factor;
+  q.r1min = p.r1;
   return q;
 }
 
@@ -226,7 +228,14 @@ struct TersoffPop {
 #pragma unroll
     for (int v = 0; v < V; ++v)
       x[v] = denormalise(ldu(ctl.pmin, v), ldu(ctl.pmax, v), ctl.genes[(size_t)p * V + v]);
-    return tersoff_lane(tersoff_params(TWO ? 2 : 1, x));
+    TLane q = tersoff_lane(tersoff_params(TWO ? 2 : 1, x));
+    // the wave's smallest R1 (every lane calls make): an atom whose bonds
+    // are all shorter has fc = 1, fc' = 0 on every lane
+    float m = q.r1;
+#pragma unroll
+    for (int o = WAVE / 2; o >= 1; o >>= 1) m = fminf(m, __shfl_xor(m, o));
+    q.r1min = m;
+    return q;
   }
 
   // One atom for 64 individuals: the geometry is wave-uniform (scalar loads
@@ -250,17 +259,29 @@ struct TersoffPop {
     const int* H = reinterpret_cast<const int*>(R);
     const int nn = ldx<U>(H, 0), unl = TWO ? ldx<U>(H, 1) : 0;
     // an atom with the full neighbour bound (every atom of diamond Si) runs
-    // a body without neighbour-count guards: no selects, no guarded loads
+    // a body without neighbour-count guards: no selects, no guarded loads;
+    // when its longest bond is also shorter than every lane's R1 (the usual
+    // case), fc = 1 and fc' = 0 exactly and the taper terms drop out
+    // (four-bond records only: the larger bodies have no registers to spare
+    // for a third copy)
     if constexpr (U) {
       if (nn == NMAX) {
-        atom_body<FORCE, U, true>(s, q, i, R, nn, unl, fblk, lane, acc);
+        if (NMAX <= 4 && __int_as_float(ldx<U>(H, 3)) < q.r1min)
+          atom_body<FORCE, U, true, NMAX <= 4>(s, q, i, R, nn, unl, fblk, lane, acc);
+        else
+          atom_body<FORCE, U, true, false>(s, q, i, R, nn, unl, fblk, lane, acc);
         return;
       }
     }
-    atom_body<FORCE, U, false>(s, q, i, R, nn, unl, fblk, lane, acc);
+    atom_body<FORCE, U, false, false>(s, q, i, R, nn, unl, fblk, lane, acc);
   }
 
-  template <bool FORCE, bool U, bool FULL>
+  // INNER: every bond inside every lane's R1.  Then fc = 1 and fc' = 0, and
+  // the body leaves out what they multiply away: fc_v beta is beta,
+  // B_v = A_v fc_v is A_v, the fc' terms of f_ij (and with them dsum) are
+  // zero.  The results are the same bits as the general body's, which
+  // multiplies by the exact 1 and adds the exact 0.
+  template <bool FORCE, bool U, bool FULL, bool INNER>
   __device__ static void atom_body(const DevSet& s, const TLane& q, int i, const float4* R,
                                    int nn, int unl, float* fblk, int lane, Acc& acc)
   {
@@ -288,8 +309,14 @@ struct TersoffPop {
 #pragma unroll
     for (int u = 0; u < NMAX; ++u) {
       if (FULL || u < nn) {
-        tersoff_fc(q, dist(u), fc[u], fcp[u]);
-        fcb[u] = fc[u] * q.beta;
+        if constexpr (INNER) {
+          fc[u] = 1.0f;
+          fcp[u] = 0.0f;
+          fcb[u] = q.beta;
+        } else {
+          tersoff_fc(q, dist(u), fc[u], fcp[u]);
+          fcb[u] = fc[u] * q.beta;
+        }
       }
     }
     // With four bonds, x = cos(theta_jk) - h and x^2 once per unordered pair:
@@ -340,12 +367,18 @@ struct TersoffPop {
         if (TWO && ((unl >> u) & 1)) tersoff_pair(q, 1, dist(u), fr, frp, fa, fap);
         else tersoff_pair(q, 0, dist(u), fr, frp, fa, fap);
         const float rep = fr - b * fa;
-        e = fmaf(fc[u], rep, e);
         const float di = dist_inv(u);
-        pf[u] = (fcp[u] * rep + fc[u] * (frp - b * fap)) * di;
-        fcp[u] *= di;
         A[u] = bp * fa;
-        B[u] = A[u] * fc[u];
+        if constexpr (INNER) {
+          e += rep;
+          pf[u] = (frp - b * fap) * di;
+          B[u] = A[u];
+        } else {
+          e = fmaf(fc[u], rep, e);
+          pf[u] = (fcp[u] * rep + fc[u] * (frp - b * fap)) * di;
+          fcp[u] *= di;
+          B[u] = A[u] * fc[u];
+        }
       }
     }
     float wxx = 0, wyy = 0, wzz = 0, wxy = 0, wyz = 0, wzx = 0;
@@ -368,17 +401,19 @@ struct TersoffPop {
           if ((FULL || v < nn) && v != u) {
             const float4 t = PRE ? row[v] : ldx<U>(R, 1 + NMAX + u * NMAX + v);
             const float x = SYM ? xsym(u, v) : t.x - q.h;
-            const float w = fmaf(A[u], fc[v], B[v]) * x;
+            const float w = (INNER ? A[u] + A[v] : fmaf(A[u], fc[v], B[v])) * x;
             cx = fmaf(w, t.y, cx);
             cy = fmaf(w, t.z, cy);
             cz = fmaf(w, t.w, cz);
-            if constexpr (SYM) dsum = fmaf(B[v], x2sym(u, v), dsum);
-            else dsum = fmaf(B[v] * x, x, dsum);
+            if constexpr (!INNER) {
+              if constexpr (SYM) dsum = fmaf(B[v], x2sym(u, v), dsum);
+              else dsum = fmaf(B[v] * x, x, dsum);
+            }
           }
         }
         const float4 r = ldx<U>(R, 1 + u);
-        const float coef = 0.5f * (pf[u] - fcp[u] * (q.beta * dsum));
-        const float hc = fc[u] * q.beta;
+        const float coef = INNER ? 0.5f * pf[u] : 0.5f * (pf[u] - fcp[u] * (q.beta * dsum));
+        const float hc = INNER ? q.beta : fc[u] * q.beta;
         const float fx = coef * r.x - hc * cx;
         const float fy = coef * r.y - hc * cy;
         const float fz = coef * r.z - hc * cz;

@@ -570,12 +570,15 @@ std::vector<Vec4> build_records(const NeighborList& nl, const Geometry& g,
     const int nn = nl.NN[i], b0 = nl.start[i];
     Vec4* R = &rec[(size_t)i * RS];
     int hdr[4] = {nn, 0, b0, 0};
+    float dmax = 0.0f; // the longest bond, in the header's last word
     for (int u = 0; u < nn; ++u) {
       hdr[1] |= (ptype[b0 + u] & 1) << u;
       R[1 + u] = g.bond[b0 + u];
+      dmax = std::max(dmax, g.bond[b0 + u].w);
       for (int w = 0; w < nn; ++w)
         if (w != u) R[1 + M + u * M + w] = g.tri[g.tri_start[i] + u * nn + w];
     }
+    std::memcpy(&hdr[3], &dmax, sizeof dmax);
     std::memcpy(&R[0], hdr, sizeof hdr);
     int* rv = reinterpret_cast<int*>(&R[1 + M + M * M]);
     for (int u = 0; u < nn; ++u) rv[u] = nl.rev[b0 + u];

@@ -86,9 +86,9 @@ std::vector<int> triplet_offsets(const NeighborList& nl, long long* total);
 std::vector<int> bond_types(const TrainSet& ts, const NeighborList& nl, bool three_body);
 // Bond slots M (4, 8 or 16) of the fixed-stride atom records of the Tersoff
 // population kernel, 0 when an atom has more than 16 neighbours; a record
-// holds 1 + M + M*M + M/2 float4: header ints (nn, species-pair bit mask,
-// first bond, 0), M bonds, M x M triplets (s, t), M reverse-bond ints and M
-// inverse distances, zero-padded.
+// holds 1 + M + M*M + M/2 float4: header (nn, species-pair bit mask, first
+// bond as ints, the longest bond's length as a float), M bonds, M x M
+// triplets (s, t), M reverse-bond ints and M inverse distances, zero-padded.
 int record_slots(int max_nn);
 // Largest outer cut-off radius R2 any individual of a minimal-Tersoff
 // potential can have: the last parameter's pmin + (pmax - pmin) g at g = 1,

@@ -50,14 +50,14 @@ def _set(two, cutoff, r1, r2, seed):
     return d, pot
 
 
-def _check(d, pot, P, seed, records):
+def _check(d, pot, P, seed, records, max_nn=28):
     """records: the atom-record bond slots the population kernel reads (0:
     the looped kernel, which reads no records)."""
     import gpuga_amd as G
     from oracle import oracle as O
     fit = G.Fitness(arrays=d)
     try:
-        assert fit.max_neighbors == 28
+        assert fit.max_neighbors == max_nn
         genes = np.random.default_rng(seed).random((P, fit.V), dtype=np.float32)
         genes[0, -1] = 1.0  # R2 at its upper bound
         f, terms = fit.compute(genes, terms=True)
@@ -104,3 +104,16 @@ def test_pruned_list_rejects_genes_above_one():
         assert np.all(np.isfinite(fit.compute(genes)))
     finally:
         fit.close()
+
+
+def test_taper_reached_by_some_lanes():
+    """R1 from 2.2 to 2.8 A: the wave's smallest R1 falls below many nearest
+    bonds (2.35 A +- displacements), so atoms take the body with the taper
+    terms, others the one without them (every bond inside every lane's R1);
+    both against the oracle."""
+    print_case("taper reached by some lanes")
+    d, pot = _set(False, 3.0, 2.8, (3.0, 3.2), seed=24)
+    d["pmin"] = d["pmin"].copy()
+    d["pmin"][-2] = 2.2
+    pot["pmin"] = d["pmin"]
+    _check(d, pot, P=70, seed=7, records=4, max_nn=4)
