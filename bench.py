@@ -241,9 +241,9 @@ def trace_split(fit, genes, n_force_cfg):
     first fetch) and idle after its last one, as fractions of waves x launch
     span."""
     rows = fit.trace(genes)
-    c, k, a0, a1, ph, slot, t0, t1, entry = (rows[:, 0], rows[:, 1], rows[:, 3], rows[:, 4],
-                                            rows[:, 5], rows[:, 6], rows[:, 7], rows[:, 8],
-                                            rows[:, 9])
+    c, k, a0, a1, ph, slot, t0, t1, entry, cb, ce = (
+        rows[:, 0], rows[:, 1], rows[:, 3], rows[:, 4], rows[:, 5], rows[:, 6], rows[:, 7],
+        rows[:, 8], rows[:, 9], rows[:, 10], rows[:, 11])
     begin = float(np.nanmin(np.where(np.isnan(entry), t0, entry)))
     end = float(t1.max())
     span = end - begin
@@ -256,9 +256,16 @@ def trace_split(fit, genes, n_force_cfg):
     out = {"span_us": span / 1e3, "waves": int(len(waves)), "items": int(len(rows))}
     for name, m in cls.items():
         atoms = float(np.sum(a1[m] - a0[m]))
+        # an item's time in three parts: set-up (item load, parameters, or
+        # the dependency wait), compute, and the rest (reductions, next fetch)
+        setup, comp, rest = cb[m] - t0[m], ce[m] - cb[m], t1[m] - ce[m]
         out[name] = {"frac": float(dur[m].sum() / total), "items": int(m.sum()),
                      "mean_item_us": float(dur[m].mean() / 1e3) if m.any() else 0.0,
-                     "us_per_atom_step": float(dur[m].sum() / max(atoms, 1) / 1e3)}
+                     "us_per_atom_step": float(dur[m].sum() / max(atoms, 1) / 1e3),
+                     "mean_setup_us": float(np.mean(setup) / 1e3) if m.any() else 0.0,
+                     "mean_compute_us": float(np.mean(comp) / 1e3) if m.any() else 0.0,
+                     "mean_after_us": float(np.mean(rest) / 1e3) if m.any() else 0.0,
+                     "compute_us_per_atom": float(comp.sum() / max(atoms, 1) / 1e3)}
     head = tail = 0.0
     for w in waves:
         m = slot == w
@@ -277,7 +284,22 @@ def trace_split(fit, genes, n_force_cfg):
     last = np.argsort(t1)[-12:]
     out["last_items"] = [{"c": int(c[i]), "k": int(k[i]), "phase": int(ph[i]),
                           "atoms": int(a1[i] - a0[i]), "start_us": float((t0[i] - begin) / 1e3),
+                          "compute_us": [float((cb[i] - begin) / 1e3),
+                                         float((ce[i] - begin) / 1e3)],
                           "end_us": float((t1[i] - begin) / 1e3)} for i in last]
+    out["compute_end_us_max"] = float((np.nanmax(ce) - begin) / 1e3)
+    # speed by dispatch round: workgroup b of the grid is resident on its CU
+    # with b mod (grid / occupancy) as the slot; round = b // (workgroups per
+    # round) orders the waves of one SIMD by age
+    wg = (slot // 4).astype(int)
+    n_wg = int(wg.max()) + 1
+    rounds = 4
+    per_round = max(1, n_wg // rounds)
+    rnd = np.minimum(wg // per_round, rounds - 1)
+    ph0 = ph == 0
+    out["us_per_atom_by_round"] = [
+        float(dur[ph0 & (rnd == r)].sum() / max(float(np.sum((a1 - a0)[ph0 & (rnd == r)])), 1) / 1e3)
+        for r in range(rounds)]
     starts = np.sort(t0 - begin)
     out["item_start_us"] = {"p90": float(np.percentile(starts, 90) / 1e3),
                             "p99": float(np.percentile(starts, 99) / 1e3),
@@ -337,6 +359,9 @@ def main():
                     help="headline line without the served-workload leg")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="generations of the per-individual-launch leg (0: none)")
+    ap.add_argument("--tail-split", type=int, default=0,
+                    help="cut configurations smaller than one item into this many "
+                         "pieces per item (0: the library default)")
     ap.add_argument("--trace-split", action="store_true",
                     help="add the work-item timeline split of one population launch")
     args = ap.parse_args()
@@ -352,7 +377,8 @@ def main():
     import gpuga_amd
     from gpuga_amd import synth
     d, w = make_workload(args.workload)
-    fit = gpuga_amd.Fitness(arrays=d, backend=1 if args.plain else 0, chunk_cost=args.chunk_cost)
+    fit = gpuga_amd.Fitness(arrays=d, backend=1 if args.plain else 0, chunk_cost=args.chunk_cost,
+                            tail_split=args.tail_split)
     P, V, N = w["P"], fit.V, fit.N
     chunk = args.chunk_cost
     if not args.plain and chunk == 0:

@@ -39,7 +39,7 @@ class Options(C.Structure):
     _fields_ = [("backend", C.c_int), ("chunk_cost", C.c_int), ("use_graph", C.c_int),
                 ("nn2b_bf16", C.c_int), ("nn2b_grid", C.c_int), ("verbose", C.c_int),
                 ("blocks_per_cu", C.c_int), ("pack_lanes", C.c_int), ("queues", C.c_int),
-                ("ga_batch", C.c_int)]
+                ("ga_batch", C.c_int), ("tail_split", C.c_int)]
 
 
 class ConvertOptions(C.Structure):
@@ -256,13 +256,14 @@ class Fitness:
         return ms.value, n.value
 
     def trace(self, genes):
-        """Work-item timeline of one population generation: array [n_items, 10]
+        """Work-item timeline of one population generation: array [n_items, 12]
         of (config, lane block, chunk, atom begin, atom end, phase, wave slot,
-        start ns, end ns, wave entry ns on the wave's first item else NaN)."""
+        start ns, end ns, wave entry ns on the wave's first item else NaN,
+        compute begin ns, compute end ns)."""
         g = _f32(genes).reshape(-1, self.V)
         n = C.c_int(0)
         _check(lib().gpuga_fitness_trace(self._h, g.shape[0], _ptr(g), None, C.byref(n)))
-        rows = np.zeros((n.value, 10))
+        rows = np.zeros((n.value, 12))
         _check(lib().gpuga_fitness_trace(self._h, g.shape[0], _ptr(g), _ptr(rows, C.c_double),
                                          C.byref(n)))
         return rows

@@ -64,6 +64,7 @@ FitnessOptions to_options(const gpuga_options* o)
   f.pack_lanes = o->pack_lanes;
   f.queues = o->queues;
   f.ga_batch = o->ga_batch;
+  f.tail_split = o->tail_split;
   return f;
 }
 
@@ -92,6 +93,7 @@ void gpuga_default_options(gpuga_options* opt)
   opt->pack_lanes = f.pack_lanes;
   opt->queues = f.queues;
   opt->ga_batch = f.ga_batch;
+  opt->tail_split = f.tail_split;
 }
 
 int gpuga_fitness_create(const char* input_dir, const gpuga_options* opt, gpuga_fitness** out)
@@ -390,8 +392,8 @@ int gpuga_fitness_trace(gpuga_fitness* h, int P, const float* genes, double* row
     need(genes, "genes");
     need(n_items, "n_items");
     const std::vector<double> r = h->f->trace(P, genes);
-    const int n = (int)(r.size() / 10);
-    if (rows) std::memcpy(rows, r.data(), sizeof(double) * 10 * std::min(n, *n_items));
+    const int n = (int)(r.size() / 12);
+    if (rows) std::memcpy(rows, r.data(), sizeof(double) * 12 * std::min(n, *n_items));
     *n_items = n;
   });
 }

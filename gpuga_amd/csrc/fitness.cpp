@@ -561,6 +561,7 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   pl.block = tersoff && loop_path_ ? WAVE : 256;
   pl.K = (P + WAVE - 1) / WAVE;
   pl.chunk_cost = chunk_cost;
+  const int tail_split = opt_.tail_split > 0 ? opt_.tail_split : 1;
   const int K = pl.K;
 
   struct Chunk {
@@ -576,6 +577,14 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
     long long total = 0;
     for (int i = a0; i < a1; ++i) total += atom_cost(spec_.type, pn.NN[i], fc);
     int n = (int)std::max<long long>(1, (total + chunk_cost - 1) / std::max(chunk_cost, 1));
+    // A configuration smaller than one item comes at the end of its queue
+    // (largest first), where the launch drains: it is cut into pieces of
+    // 1/tail_split of an item, so that the last items are short.  (Not the
+    // three-body force configurations: they come first.)
+    if (total < chunk_cost && !(tersoff && fc)) {
+      const long long piece = std::max<long long>(1, chunk_cost / std::max(tail_split, 1));
+      n = (int)std::max<long long>(1, (total + piece - 1) / piece);
+    }
     n = std::min(n, a1 - a0);
     nch[c] = n;
     slot0[c] = slots;
@@ -637,15 +646,18 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
     for (int k = 0; k < K; ++k) {
       const int G = k == K - 1 ? G_last : 1;
       const int n = ch.a1 - ch.a0, steps = (n + G - 1) / G;
+      // (with the assembly of a one-chunk three-body force configuration)
+      const long long asm_cost = tersoff && ch.c < ts_.Nc_force && nch[ch.c] == 1 ? 8LL * steps : 0;
       eval.push_back(Costed{WorkItem{ch.c, k, ch.chunk, ch.a0, ch.a1, ch.cls, 0, G},
-                            ch.cost * steps / n});
+                            ch.cost * steps / n + asm_cost});
     }
-  // three-body force configurations: one force-assembly item per chunk and
-  // lane block
+  // three-body force configurations of several chunks: one force-assembly
+  // item per chunk and lane block (a configuration of one chunk is
+  // assembled by its phase-0 item)
   std::vector<Costed> assemble;
   if (tersoff)
     for (const Chunk& ch : chunks)
-      if (ch.c < ts_.Nc_force)
+      if (ch.c < ts_.Nc_force && nch[ch.c] > 1)
         for (int k = 0; k < K; ++k) {
           const int G = k == K - 1 ? G_last : 1;
           const int n = ch.a1 - ch.a0, steps = (n + G - 1) / G;
@@ -1261,7 +1273,8 @@ std::vector<double> Fitness::trace(int P, const float* genes)
   PopPlan& pl = plan(P);
   hipStream_t st = stream_;
   const size_t n = pl.n_items;
-  DeviceBuffer<unsigned long long> buf(std::max<size_t>(4 * n, 1));
+  const size_t F = kTraceFields;
+  DeviceBuffer<unsigned long long> buf(std::max<size_t>(F * n, 1));
   buf.zero(st);
   std::vector<WorkItem> items(n);
   pl.items.download(items.data(), n, st);
@@ -1270,7 +1283,7 @@ std::vector<double> Fitness::trace(int P, const float* genes)
   pl.ctl.trace = buf.get();
   launch_population(pl, st);
   pl.ctl.trace = nullptr;
-  std::vector<unsigned long long> t(4 * n);
+  std::vector<unsigned long long> t(F * n);
   buf.download(t.data(), t.size(), st);
   unsigned status = 0;
   GPUGA_HIP(hipMemcpyAsync(&status, pl.ctl.status, sizeof status, hipMemcpyDeviceToHost, st));
@@ -1280,15 +1293,17 @@ std::vector<double> Fitness::trace(int P, const float* genes)
   GPUGA_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_));
   const double ns_per_tick = 1e6 / std::max(khz, 1);
   unsigned long long t0 = ~0ull;
-  for (size_t i = 0; i < n; ++i) t0 = std::min(t0, t[4 * i + 1]);
+  for (size_t i = 0; i < n; ++i) t0 = std::min(t0, t[F * i + 1]);
   auto rel = [&](unsigned long long x) { return ((double)x - (double)t0) * ns_per_tick; };
   std::vector<double> rows;
-  rows.reserve(10 * n);
+  rows.reserve(12 * n);
   for (size_t i = 0; i < n; ++i) {
     const WorkItem& w = items[i];
+    const unsigned long long* x = &t[F * i];
     rows.insert(rows.end(), {(double)w.c, (double)w.k, (double)w.chunk, (double)w.a0,
-                             (double)w.a1, (double)w.phase, (double)t[4 * i], rel(t[4 * i + 1]),
-                             rel(t[4 * i + 2]), t[4 * i + 3] ? rel(t[4 * i + 3]) : NAN});
+                             (double)w.a1, (double)w.phase, (double)x[0], rel(x[1]), rel(x[2]),
+                             x[3] ? rel(x[3]) : NAN, x[4] ? rel(x[4]) : NAN,
+                             x[5] ? rel(x[5]) : NAN});
   }
   return rows;
 }

@@ -26,6 +26,8 @@ struct FitnessOptions {
   int pack_lanes = 1;     // pack a last lane block of <= 32 individuals
   int queues = 0;         // work queues of the population launch; 0 = one per XCD
   int ga_batch = 0;       // generations per staged batch of the device GA; 0 = up to 32
+  int tail_split = 0;     // configurations smaller than one work item are cut into
+                          // pieces of 1/tail_split of an item (0, 1: not cut)
 };
 
 // Device-resident copy of one training set and its geometry tables.
@@ -121,10 +123,10 @@ public:
   // first_generation; ga holds the state before and after.  best[g] and
   // elite[g][V] (genes in [0, 1]) receive each generation's ga.out record.
   void run_ga(GA& ga, int first_generation, int generations, float* best, float* elite);
-  // One generation with the item timeline recorded: rows of 10 values per
+  // One generation with the item timeline recorded: rows of 12 values per
   // work item (config, lane block, chunk, atom begin, atom end, phase, wave
   // slot, start ns, end ns, entry ns of the wave for its first item else
-  // NaN; times relative to the first start).
+  // NaN, compute begin and end ns; times relative to the first start).
   std::vector<double> trace(int P, const float* genes);
   // Work-item size from the training set alone (no timing; deterministic).
   int heuristic_chunk_cost(int P) const;

@@ -17,7 +17,9 @@
 // lane blocks of a chunk together) evaluate the potential.  For three-body potentials the force on an atom needs the
 // partial forces of its neighbours, so phase-0 items of force configurations
 // only publish per-bond forces, and phase-1 items assemble F_i - F_ref,i for
-// their chunk once all chunks of their configuration have published.  A
+// their chunk once all chunks of their configuration have published (a
+// configuration of one chunk is assembled by the wave that evaluated it,
+// right after, with no phase-1 item).  A
 // phase-1 item sits in its configuration's queue after every phase-0 item of
 // that configuration (about one item per resident wave of the queue later,
 // so that they have finished by then).  Items of a queue are handed out in
@@ -52,6 +54,7 @@
 namespace gpuga {
 
 constexpr int CFG_GROUP = 64; // configurations per reduction group
+constexpr int kTraceFields = 6; // words per item of the optional timeline
 constexpr int kQueuesPerXcd = 4; // work queues per XCD
 constexpr int kXcds = 8;         // XCDs of an MI355X
 constexpr int kQueues = kQueuesPerXcd * kXcds;
@@ -99,8 +102,9 @@ struct PopCtl {
   float* fitness;         // [P]
   double* terms;          // [P][3]
   FitnessNorm norm;
-  // optional item timeline: [n_items][4] = wave slot, start, end, 0 in
-  // wall-clock ticks (constant rate); null when not recording
+  // optional item timeline: [n_items][kTraceFields] = wave slot, start,
+  // end, the wave's entry (first item, else 0), compute begin, compute end
+  // in wall-clock ticks (constant rate); null when not recording
   unsigned long long* trace;
 };
 
@@ -289,12 +293,13 @@ __device__ __forceinline__ unsigned xcc_id()
 
 // Next item for this wave: from queue q while it has items, then from the
 // next queue (cyclically) that is still open; -1 once every queue is
-// closed.  The fetch that first runs past the end of a queue marks it closed
-// (one atomic OR per queue and launch); the others read the closed mask when
-// their fetch comes back empty, so a wave finds an open queue, or learns that
-// all are closed, with one load instead of trying the queues one by one.
-// Counters are kFetchStride words apart, each in its own memory channel's
-// lines.
+// closed.  The wave that takes the last item of a queue marks it closed (one
+// atomic OR per queue and launch, issued without waiting for it); a fetch
+// that comes back empty reads the closed mask, so a wave finds an open
+// queue, or learns that all are closed, with one load instead of trying the
+// queues one by one.  (The first fetch past the end marks the queue as well,
+// for a fetch that overtakes the OR.)  Counters are kFetchStride words
+// apart, each in its own memory channel's lines.
 __device__ inline int next_item(const PopCtl& ctl, int& q)
 {
   const unsigned long long all = ctl.nq >= 64 ? ~0ull : (1ull << ctl.nq) - 1ull;
@@ -302,7 +307,10 @@ __device__ inline int next_item(const PopCtl& ctl, int& q)
     unsigned* f = ctl.fetch + (long)q * kFetchStride;
     const unsigned q0 = (unsigned)ldu(ctl.qstart, q), len = (unsigned)ldu(ctl.qstart, q + 1) - q0;
     const unsigned it = wave_atomic_add(f, 1u);
-    if (it < len) return (int)(q0 + it);
+    if (it < len) {
+      if (it == len - 1u && __lane_id() == 0) atomicOr(ctl.closed, 1ull << q);
+      return (int)(q0 + it);
+    }
     unsigned long long closed = 0;
     if (__lane_id() == 0) {
       if (it == len) closed = atomicOr(ctl.closed, 1ull << q) | (1ull << q);
@@ -316,6 +324,31 @@ __device__ inline int next_item(const PopCtl& ctl, int& q)
     const unsigned long long above = q >= 63 ? 0ull : open & (~0ull << (q + 1));
     q = __builtin_ctzll(above ? above : open);
   }
+}
+
+// When several waves of a SIMD are ready, the oldest issues first, so in a
+// persistent launch the youngest wave of each SIMD runs all of its items
+// slowest (measured: about 1.5 us per atom for the first workgroups
+// dispatched to a CU against 2.2 us for the last, with four per CU), and
+// its large items end the launch.  Kernels call this once per atom with a
+// running count: the wave priority then cycles through its four levels, at
+// a phase set by the workgroup's dispatch round, so the waves of a SIMD take
+// turns at the top.
+__device__ __forceinline__ void rotate_priority(unsigned step)
+{
+  switch (step & 3u) {
+  case 0: __builtin_amdgcn_s_setprio(0); break;
+  case 1: __builtin_amdgcn_s_setprio(1); break;
+  case 2: __builtin_amdgcn_s_setprio(2); break;
+  default: __builtin_amdgcn_s_setprio(3); break;
+  }
+}
+
+// The dispatch round of this workgroup among those sharing its CU (grids are
+// sized to whole rounds of the resident workgroups).
+__device__ __forceinline__ unsigned dispatch_round()
+{
+  return (blockIdx.x * 4u) / gridDim.x;
 }
 
 // Sum over the G lane groups of a packed item: lanes l, l ^ W, l ^ 2W, ...
@@ -358,17 +391,22 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
                 (unsigned)ctl.nq);
   for (;;) {
     const int it = next_item(ctl, q);
+    unsigned long long* tr = ctl.trace ? ctl.trace + (long)kTraceFields * it : nullptr;
     if (ctl.trace && lane == 0) {
       const unsigned long long now = wall_clock64();
-      if (prev != 0xffffffffu) ctl.trace[4 * (long)prev + 2] = now;
+      if (prev != 0xffffffffu) ctl.trace[(long)kTraceFields * prev + 2] = now;
       if (it >= 0) {
-        ctl.trace[4 * (long)it + 0] = wave_slot;
-        ctl.trace[4 * (long)it + 1] = now;
+        tr[0] = wave_slot;
+        tr[1] = now;
         // the wave's entry time on its first item (fetch latency), else 0
-        ctl.trace[4 * (long)it + 3] = prev == 0xffffffffu ? t_entry : 0ull;
+        tr[3] = prev == 0xffffffffu ? t_entry : 0ull;
       }
       prev = (unsigned)it;
     }
+    // marks of the item's compute phase (lane 0, recording only)
+    auto mark = [&](int field) {
+      if (tr && lane == 0) tr[field] = wall_clock64();
+    };
     if (it < 0) {
       // Every queue is empty.  The last wave to finish returns the queue
       // counters to zero (all other waves have stopped fetching by then).
@@ -397,7 +435,9 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
           const int G = item.groups, W = WAVE / G;
           const int col = lane & (W - 1);
           const typename Pot::Lane q = Pot::make(ctl, min(k * WAVE + col, ctl.P - 1));
+          mark(4);
           Pot::chunk_packed(set, q, item, ctl, ex, col, lane / W, G, acc);
+          mark(5);
           if (!(Pot::kForcePass && is_force)) {
 #pragma unroll
             for (int u = 0; u < 8; ++u) acc.v[u] = group_sum(acc.v[u], W);
@@ -407,14 +447,30 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
         // individual of this lane; lanes past P repeat the last one, write nothing
         const int p = min(k * WAVE + lane, ctl.P - 1);
         const typename Pot::Lane q = Pot::make(ctl, p);
+        mark(4);
         Pot::chunk(set, q, item, ctl, ex, acc);
+        mark(5);
       }
       if (Pot::kForcePass && is_force) {
         publish(); // the per-bond forces of this chunk are complete
-        wave_atomic_add(cnt, 1u);
-        continue;
-      }
-      if (nch > 1) {
+        if (nch > 1) {
+          wave_atomic_add(cnt, 1u);
+          continue;
+        }
+        // The item was the whole configuration: this wave assembles its
+        // forces at once, from its own completed stores (no assembly item,
+        // no wait).
+        if (packed) {
+          if constexpr (can_pack) {
+            const int G = item.groups, W = WAVE / G;
+            acc.v[7] = group_sum(Pot::force_pass_packed(set, ctl, item.a0, item.a1, k,
+                                                        lane & (W - 1), lane / W, G),
+                                 W);
+          }
+        } else {
+          acc.v[7] = Pot::force_pass(set, ctl, ex, item.a0, item.a1, k, lane);
+        }
+      } else if (nch > 1) {
         if (!fold_chunks(ctl, cnt, c, k, item.chunk, nch, lane, acc)) continue;
         if (lane == 0) *cnt = 0u;
       }
@@ -422,6 +478,7 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
       if (!wave_wait_count(cnt, (unsigned)nch, ctl.spin_limit)) {
         if (lane == 0) atomicOr(ctl.status, 1u);
       }
+      mark(4);
       if (packed) {
         if constexpr (can_pack) {
           const int G = item.groups, W = WAVE / G;
@@ -432,6 +489,7 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
       } else {
         acc.v[7] = Pot::force_pass(set, ctl, ex, item.a0, item.a1, k, lane);
       }
+      mark(5);
       unsigned* cnt2 = &ctl.cnt2[(long)c * ctl.K + k];
       if (nch > 1) {
         if (!fold_chunks(ctl, cnt2, c, k, item.chunk, nch, lane, acc)) continue;

@@ -461,11 +461,18 @@ struct TersoffPop {
                                const PopCtl& ctl, int, Acc& acc)
   {
     const int lane = __lane_id();
+    const unsigned r = dispatch_round();
     if (it.c < s.Nc_force) {
       float* fblk = ctl.bondf + (size_t)it.k * ctl.B_force * 3 * WAVE;
-      for (int i = it.a0; i < it.a1; ++i) atom<true, true>(s, q, i, fblk, lane, acc);
+      for (int i = it.a0; i < it.a1; ++i) {
+        rotate_priority(r + (unsigned)i);
+        atom<true, true>(s, q, i, fblk, lane, acc);
+      }
     } else {
-      for (int i = it.a0; i < it.a1; ++i) atom<false, true>(s, q, i, nullptr, lane, acc);
+      for (int i = it.a0; i < it.a1; ++i) {
+        rotate_priority(r + (unsigned)i);
+        atom<false, true>(s, q, i, nullptr, lane, acc);
+      }
     }
   }
 
@@ -495,10 +502,10 @@ struct TersoffPop {
     const float* fblk = ctl.bondf + (size_t)k * ctl.B_force * 3 * WAVE + lane;
     const long N = s.N;
     double err = 0.0;
-    // two atoms per step, every load unconditional, so all of them are in
-    // flight together (the loads go to memory: their latency, not their
-    // number, sets the time; four per step would cost the kernel a wave)
-    constexpr int STEP = 2;
+    // four atoms per step, every load unconditional, so all of them (60)
+    // are in flight together: the loads go to memory, and their latency,
+    // not their number, sets the time
+    constexpr int STEP = 4;
     for (int i = a0; i < a1; i += STEP) {
       float d[STEP][3];
 #pragma unroll

@@ -35,6 +35,8 @@ typedef struct gpuga_options {
   int queues;        /* work queues of the population launch; 0 = one per XCD */
   int ga_batch;      /* generations per staged batch of gpuga_fitness_run_ga's
                         random decisions; 0 = up to 32 */
+  int tail_split;    /* configurations smaller than one work item are cut into
+                        pieces of 1/tail_split of an item (0, 1: not cut) */
 } gpuga_options;
 
 /* Thread-local text of the last failure (src/error.cu:21-25 prints it). */
@@ -154,10 +156,13 @@ int gpuga_fitness_set_timing(gpuga_fitness* h, int on);
 int gpuga_fitness_timing(gpuga_fitness* h, double* kernel_ms, long* launches, int reset);
 
 /* One population generation with its work-item timeline recorded (no
- * reference counterpart; a profiling aid).  rows[n_items][10] = configuration,
- * lane block, chunk, first atom, end atom, phase, wave slot, start ns, end ns,
- * and for a wave's first item the wave's entry ns (else NaN); times relative
- * to the first start.  Call with rows = NULL to get *n_items. */
+ * reference counterpart; a profiling aid).  rows[n_items][12] = configuration,
+ * lane block, chunk, first atom, end atom, phase, wave slot, start ns, end ns
+ * (when the wave's next fetch returned), for a wave's first item the wave's
+ * entry ns (else NaN), then the ns at which the item's compute began (its
+ * parameters set up, or its dependency wait over) and ended (before the
+ * reductions); times relative to the first start.  Call with rows = NULL to
+ * get *n_items. */
 int gpuga_fitness_trace(gpuga_fitness* h, int P, const float* genes, double* rows, int* n_items);
 
 /* Polls of the population kernel's dependency wait (a force-assembly item
