@@ -1358,13 +1358,23 @@ int Fitness::autotune(int P)
   cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
   int best = cands.back();
   double best_ms = 1e30;
-  for (int c : cands) {
-    const double ms = time_plan(P, c, 3);
+  auto consider = [&](int c, int reps) {
+    const double ms = time_plan(P, c, reps);
     if (opt_.verbose) std::printf("autotune P=%d chunk_cost=%d: %.4f ms\n", P, c, ms);
     if (ms < best_ms) {
       best_ms = ms;
       best = c;
     }
+  };
+  for (int c : cands) consider(c, 3);
+  // then the neighbourhood of the best (the coarse candidates are a factor
+  // of two apart, and launch times differ by about 1% from run to run), the
+  // best one timed again alongside
+  const int coarse = best;
+  best_ms = 1e30;
+  for (double f : {0.6, 0.75, 0.87, 1.0, 1.15, 1.33, 1.6}) {
+    const long long c = std::llround(coarse * f);
+    if (c >= 64 && c <= std::max<long long>(maxcfg, 64)) consider((int)c, 6);
   }
   tuned_[P] = best;
   return best;

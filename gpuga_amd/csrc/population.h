@@ -189,10 +189,22 @@ __device__ inline void sum3(const double* e0, const double* e1, const double* e2
       s2[u & 3] += v2[u];
     }
   }
-  for (; c < n; ++c) {
-    s0[0] += cload(e0 + c * stride);
-    s1[0] += cload(e1 + c * stride);
-    s2[0] += cload(e2 + c * stride);
+  // the rest: all loads first as well
+  if (c < n) {
+    double v0[B], v1[B], v2[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const bool in = c + u < n;
+      v0[u] = in ? cload(e0 + (c + u) * stride) : 0.0;
+      v1[u] = in ? cload(e1 + (c + u) * stride) : 0.0;
+      v2[u] = in ? cload(e2 + (c + u) * stride) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      s0[u & 3] += v0[u];
+      s1[u & 3] += v1[u];
+      s2[u & 3] += v2[u];
+    }
   }
   out[0] = (s0[0] + s0[1]) + (s0[2] + s0[3]);
   out[1] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
