@@ -1196,11 +1196,13 @@ void Fitness::predict(const std::string& dir, const float* elite)
     std::fprintf(ff, "%g %g %g %g %g %g\n", f[n], f[n + N], f[n + N * 2], ts_.force_ref[n],
                  ts_.force_ref[n + N], ts_.force_ref[n + N * 2]);
   std::fclose(ff);
+  // per-configuration sums in FP32, atom by atom, as the reference writes
+  // them (src/fitness.cu:335-345): equal per-atom values give the same text
   auto per_config = [&](FILE* fid, const float* data, const float* ref) {
     for (int c = ts_.Nc_force; c < ts_.Nc; ++c) {
-      double sum = 0.0;
+      float sum = 0.0f;
       for (int m = 0; m < ts_.Na[c]; ++m) sum += data[ts_.Na_sum[c] + m];
-      std::fprintf(fid, "%g %g\n", (float)sum, ref[c]);
+      std::fprintf(fid, "%g %g\n", sum, ref[c]);
     }
   };
   FILE* fe = open("energy.out");

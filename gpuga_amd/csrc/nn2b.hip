@@ -83,6 +83,7 @@ __global__ void __launch_bounds__(256) nn2b_tabulate_f32(PopCtl ctl, NN2BGrid gr
 
 struct NN2BPop {
   static constexpr bool kForcePass = false;
+  static constexpr int kFoldValues = 12;
   static constexpr bool kPacked = false;
   struct Lane {
     int dummy;

@@ -137,6 +137,7 @@ __device__ __forceinline__ void ri_pair_pop(const RILane& q, float d, const floa
 
 struct RIPop {
   static constexpr bool kForcePass = false;
+  static constexpr int kFoldValues = 12;
   static constexpr bool kPacked = false;
   using Lane = RILane;
   using Extra = const float4*;

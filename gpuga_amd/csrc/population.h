@@ -164,64 +164,52 @@ __device__ inline bool wave_wait_count(const unsigned* p, unsigned target, int l
   }
 }
 
-// Sum n rows (stride `stride` doubles) of three arrays in a fixed order.  The
+// Sum n rows (stride `stride` doubles) of NA arrays in a fixed order.  The
 // rows are read from memory (coherent loads), so the time is the number of
-// round trips: the loads of 8 rows (24 values) are all issued before any is
-// added, then the next 8, with 4 partial sums per array.
-__device__ inline void sum3(const double* e0, const double* e1, const double* e2, long stride,
-                            int n, double out[3])
+// round trips: VALS values (VALS / NA rows) are loaded before any is added,
+// then the next VALS, into 4 partial sums per array.  (VALS sets the
+// registers this needs: two per value.)
+template <int NA, int VALS>
+__device__ inline void sum_rows(const double* const (&a)[NA], long stride, int n, double out[NA])
 {
-  constexpr int B = 8;
-  double s0[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
-  int c = 0;
-  for (; c + B <= n; c += B) {
-    double v0[B], v1[B], v2[B];
+  constexpr int B = VALS / NA;
+  double s[NA][4];
 #pragma unroll
-    for (int u = 0; u < B; ++u) {
-      v0[u] = cload(e0 + (c + u) * stride);
-      v1[u] = cload(e1 + (c + u) * stride);
-      v2[u] = cload(e2 + (c + u) * stride);
-    }
+  for (int x = 0; x < NA; ++x)
 #pragma unroll
-    for (int u = 0; u < B; ++u) {
-      s0[u & 3] += v0[u];
-      s1[u & 3] += v1[u];
-      s2[u & 3] += v2[u];
-    }
+    for (int u = 0; u < 4; ++u) s[x][u] = 0.0;
+  for (int c = 0; c < n; c += B) {
+    double v[NA][B];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+#pragma unroll
+      for (int x = 0; x < NA; ++x) v[x][u] = c + u < n ? cload(a[x] + (c + u) * stride) : 0.0;
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+#pragma unroll
+      for (int x = 0; x < NA; ++x) s[x][u & 3] += v[x][u];
   }
-  // the rest: all loads first as well
-  if (c < n) {
-    double v0[B], v1[B], v2[B];
 #pragma unroll
-    for (int u = 0; u < B; ++u) {
-      const bool in = c + u < n;
-      v0[u] = in ? cload(e0 + (c + u) * stride) : 0.0;
-      v1[u] = in ? cload(e1 + (c + u) * stride) : 0.0;
-      v2[u] = in ? cload(e2 + (c + u) * stride) : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < B; ++u) {
-      s0[u & 3] += v0[u];
-      s1[u & 3] += v1[u];
-      s2[u & 3] += v2[u];
-    }
-  }
-  out[0] = (s0[0] + s0[1]) + (s0[2] + s0[3]);
-  out[1] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
-  out[2] = (s2[0] + s2[1]) + (s2[2] + s2[3]);
+  for (int x = 0; x < NA; ++x) out[x] = (s[x][0] + s[x][1]) + (s[x][2] + s[x][3]);
 }
 
-// Configuration c of lane block k has its three error terms: propagate
+// Configuration c of lane block k has its error terms (force configurations
+// the force term eF only, the others energy and virial eE, eW): propagate
 // through the group and lane-block reductions, writing the fitness when the
-// lane block is complete.
-__device__ inline void config_done(const PopCtl& ctl, int Nc, int c, int k, int lane, double eE,
-                                   double eW, double eF)
+// lane block is complete.  Only the terms a configuration has are stored
+// and read back.
+template <int VALS>
+__device__ inline void config_done(const PopCtl& ctl, int Nc, int Nc_force, int c, int k,
+                                   int lane, double eE, double eW, double eF)
 {
   const long E3 = (long)Nc * WAVE;
   double* e = ctl.err + (long)k * 3 * E3 + (long)c * WAVE + lane;
-  cstore(e, eE);
-  cstore(e + E3, eW);
-  cstore(e + 2 * E3, eF);
+  if (c < Nc_force) {
+    cstore(e + 2 * E3, eF);
+  } else {
+    cstore(e, eE);
+    cstore(e + E3, eW);
+  }
   const int groups = (Nc + CFG_GROUP - 1) / CFG_GROUP;
   const int g = c / CFG_GROUP;
   const int g0 = g * CFG_GROUP, gn = min(CFG_GROUP, Nc - g0);
@@ -230,9 +218,15 @@ __device__ inline void config_done(const PopCtl& ctl, int Nc, int c, int k, int 
   if (gd != (unsigned)gn - 1u) return;
   observe();
   if (lane == 0) ctl.grp_done[k * groups + g] = 0u;
-  const double* b = ctl.err + (long)k * 3 * E3 + (long)g0 * WAVE + lane;
-  double s[3];
-  sum3(b, b + E3, b + 2 * E3, WAVE, gn, s);
+  // the group's force configurations [g0, fe), then its others [fe, g0 + gn)
+  const int fe = min(max(Nc_force, g0), g0 + gn);
+  const double* b = ctl.err + (long)k * 3 * E3 + lane;
+  double s[3], f[1];
+  const double* const ef[1] = {b + 2 * E3 + (long)g0 * WAVE};
+  sum_rows<1, VALS>(ef, WAVE, fe - g0, f);
+  const double* const ee[2] = {b + (long)fe * WAVE, b + E3 + (long)fe * WAVE};
+  sum_rows<2, VALS>(ee, WAVE, g0 + gn - fe, s);
+  s[2] = f[0];
   double* gp = ctl.gpart + ((long)k * groups + g) * 3 * WAVE + lane;
   cstore(gp, s[0]);
   cstore(gp + WAVE, s[1]);
@@ -243,7 +237,8 @@ __device__ inline void config_done(const PopCtl& ctl, int Nc, int c, int k, int 
   observe();
   if (lane == 0) ctl.blk_done[k] = 0u;
   const double* gb = ctl.gpart + (long)k * groups * 3 * WAVE + lane;
-  sum3(gb, gb + WAVE, gb + 2 * WAVE, 3 * WAVE, groups, s);
+  const double* const eg[3] = {gb, gb + WAVE, gb + 2 * WAVE};
+  sum_rows<3, VALS>(eg, 3 * WAVE, groups, s);
   const double tE = sqrt(s[0] * ctl.norm.inv_pss);
   const double tW = sqrt(s[1] * ctl.norm.inv_vss);
   const double tF = sqrt(s[2] * ctl.norm.inv_fss);
@@ -259,37 +254,51 @@ __device__ inline void config_done(const PopCtl& ctl, int Nc, int c, int k, int 
 
 // Store a chunk partial and, for the last chunk to arrive, fold all chunk
 // partials of (c, k) in chunk order into acc.  Returns false for the others.
+// A force configuration's partial is its force-error sum alone (acc.v[7]),
+// another configuration's its energy and virial (acc.v[0..6]).
+template <int VALS>
 __device__ inline bool fold_chunks(const PopCtl& ctl, unsigned* counter, int c, int k, int chunk,
-                                   int nch, int lane, Acc& acc)
+                                   int nch, int lane, bool force, Acc& acc)
 {
   const long slot0 = ldu(ctl.cfg_slot0, c);
-  double* slot = ctl.part + ((slot0 + chunk) * ctl.K + k) * 8 * WAVE;
+  double* slot = ctl.part + ((slot0 + chunk) * ctl.K + k) * 8 * WAVE + lane;
+  if (force) {
+    cstore(slot + 7 * WAVE, acc.v[7]);
+  } else {
 #pragma unroll
-  for (int u = 0; u < 8; ++u) cstore(slot + u * WAVE + lane, acc.v[u]);
+    for (int u = 0; u < 7; ++u) cstore(slot + u * WAVE, acc.v[u]);
+  }
   publish();
   const unsigned done = wave_atomic_add(counter, 1u);
   if (done != (unsigned)nch - 1u) return false;
   observe();
+  const double* base = ctl.part + (slot0 * ctl.K + k) * 8 * WAVE + lane;
+  const long stride = (long)ctl.K * 8 * WAVE; // one chunk to the next
 #pragma unroll
   for (int u = 0; u < 8; ++u) acc.v[u] = 0.0;
-  // two chunks' partials (16 values) loaded per round trip, added in chunk order
-  int ch = 0;
-  for (; ch + 2 <= nch; ch += 2) {
-    const double* s = ctl.part + ((slot0 + ch) * ctl.K + k) * 8 * WAVE;
-    const double* t = s + (long)ctl.K * 8 * WAVE;
-    double a[8], b[8];
+  if (force) {
+    // VALS chunks' force sums per round trip, added in chunk order
+    constexpr int B = VALS;
+    for (int ch = 0; ch < nch; ch += B) {
+      double a[B];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      a[u] = cload(s + u * WAVE + lane);
-      b[u] = cload(t + u * WAVE + lane);
+      for (int j = 0; j < B; ++j)
+        a[j] = ch + j < nch ? cload(base + (ch + j) * stride + 7 * WAVE) : 0.0;
+#pragma unroll
+      for (int j = 0; j < B; ++j) acc.v[7] += a[j];
     }
+  } else {
+    // two chunks' partials (14 values) per round trip, added in chunk order
+    for (int ch = 0; ch < nch; ch += 2) {
+      double a[7], b[7];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc.v[u] = (acc.v[u] + a[u]) + b[u];
-  }
-  if (ch < nch) {
-    const double* s = ctl.part + ((slot0 + ch) * ctl.K + k) * 8 * WAVE;
+      for (int u = 0; u < 7; ++u) {
+        a[u] = cload(base + ch * stride + u * WAVE);
+        b[u] = ch + 1 < nch ? cload(base + (ch + 1) * stride + u * WAVE) : 0.0;
+      }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc.v[u] += cload(s + u * WAVE + lane);
+      for (int u = 0; u < 7; ++u) acc.v[u] = (acc.v[u] + a[u]) + b[u];
+    }
   }
   return true;
 }
@@ -378,6 +387,8 @@ __device__ inline double group_sum(double x, int W)
 //   static constexpr bool kForcePass;               three-body force assembly
 //   static double force_pass(set, ctl, ex, a0, a1, k, lane)
 //                                                   sum |F-F_ref|^2 over [a0, a1)
+//   static constexpr int kFoldValues;               values loaded per round trip
+//                                                   by the reductions (registers)
 //   static constexpr bool kPacked;                  packed items supported:
 //   static void chunk_packed(set, q, item, ctl, ex, col, g, G, Acc&)
 //   static double force_pass_packed(set, ctl, a0, a1, k, col, g, G)
@@ -483,7 +494,8 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
           acc.v[7] = Pot::force_pass(set, ctl, ex, item.a0, item.a1, k, lane);
         }
       } else if (nch > 1) {
-        if (!fold_chunks(ctl, cnt, c, k, item.chunk, nch, lane, acc)) continue;
+        if (!fold_chunks<Pot::kFoldValues>(ctl, cnt, c, k, item.chunk, nch, lane, is_force, acc))
+          continue;
         if (lane == 0) *cnt = 0u;
       }
     } else {
@@ -504,7 +516,8 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
       mark(5);
       unsigned* cnt2 = &ctl.cnt2[(long)c * ctl.K + k];
       if (nch > 1) {
-        if (!fold_chunks(ctl, cnt2, c, k, item.chunk, nch, lane, acc)) continue;
+        if (!fold_chunks<Pot::kFoldValues>(ctl, cnt2, c, k, item.chunk, nch, lane, true, acc))
+          continue;
       }
       if (lane == 0) {
         *cnt = 0u;
@@ -524,7 +537,7 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
         eW += dW * dW;
       }
     }
-    config_done(ctl, Nc, c, k, lane, eE, eW, eF);
+    config_done<Pot::kFoldValues>(ctl, Nc, set.Nc_force, c, k, lane, eE, eW, eF);
   }
 }
 

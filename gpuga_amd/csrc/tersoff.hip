@@ -218,6 +218,7 @@ void launch_tersoff_plain(const DevSet& s, const TersoffParams& q, float* b, flo
 template <int NMAX, bool TWO>
 struct TersoffPop {
   static constexpr bool kForcePass = true;
+  static constexpr int kFoldValues = 24;
   using Lane = TLane;
   using Extra = int;
 
@@ -574,6 +575,7 @@ struct TersoffPop {
 template <bool TWO>
 struct TersoffLoop {
   static constexpr bool kForcePass = true;
+  static constexpr int kFoldValues = 24;
   static constexpr bool kPacked = false;
   using Lane = TLane;
   using Extra = LoopTab;
