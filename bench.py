@@ -16,7 +16,8 @@ individual x atom fitness evaluations per second.
                   [--no-cpu-baseline] [--plain] [--host-ga]
 
 Extra workloads (not the headline line): tersoff_small, rigid_ion, nn2b,
-serve (16 mixed jobs, Poisson arrivals).
+nn2b_species (NN2B with the species pair as a network input), serve (16 mixed
+jobs, Poisson arrivals).
 """
 import argparse
 import json
@@ -55,6 +56,8 @@ WORKLOADS = {
     "tersoff_small": dict(kind="tersoff", Nc=60, sizes="fixed", P=50, seed=1),
     "rigid_ion": dict(kind="ri", Nc=500, P=200, seed=3),
     "nn2b": dict(kind="nn2b", Nc=800, P=256, seed=5),
+    # the same with the species pair as a second network input (type 4)
+    "nn2b_species": dict(kind="nn2b", Nc=800, P=256, seed=5, species=True),
 }
 
 
@@ -67,7 +70,7 @@ def make_workload(name):
     elif w["kind"] == "ri":
         d = synth.fluorite_ab2(Nc=w["Nc"], seed=w["seed"])
     else:
-        d = synth.wurtzite_ab(Nc=w["Nc"], seed=w["seed"])
+        d = synth.wurtzite_ab(Nc=w["Nc"], seed=w["seed"], species=w.get("species", False))
     unlabelled = gpuga_amd.Fitness(arrays=d)
     d = synth.label(d, unlabelled.find_force)
     unlabelled.close()

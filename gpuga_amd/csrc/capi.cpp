@@ -134,8 +134,10 @@ int gpuga_fitness_create_arrays(int potential_type, int num_neurons, float cutof
     case POT_TERSOFF1: s.V = 9; break;
     case POT_TERSOFF2: s.V = 13; break;
     case POT_NN2B:
+    case POT_NN2B_SPECIES:
       if (num_neurons < 1) fail("num_neurons should >= 1.");
-      s.V = num_neurons * (num_neurons + 4) + 1;
+      s.V = num_neurons * (num_neurons + 4) + 1 +
+            (potential_type == POT_NN2B_SPECIES ? num_neurons : 0);
       break;
     default: fail("unsupported potential type.");
     }
@@ -148,7 +150,7 @@ int gpuga_fitness_create_arrays(int potential_type, int num_neurons, float cutof
     s.pmax.resize(s.V);
     s.names.resize(s.V);
     for (int v = 0; v < s.V; ++v) {
-      if (potential_type == POT_NN2B) {
+      if (s.nn2b()) {
         s.pmin[v] = -3.0f;
         s.pmax[v] = 3.0f;
       } else {

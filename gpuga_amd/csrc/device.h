@@ -42,7 +42,8 @@ struct DevSet {
   //        (d^-6, 6 d^-7, erfc(a d)/d, -(d/dr)(erfc(a d)/d)).
   //  NN2B: per atom {first pair, pairs, 0, 0} (ordered by d); per pair three
   //        float4: (unit vector, d), cubic Hermite weights on the tabulation
-  //        grid, (fc, fc', grid cell as int bits, 0).
+  //        grid, (fc, fc', grid cell as int bits, species code t_i + t_j as
+  //        int bits with the species-pair input, else 0).
   const int4* pair_hdr;
   const float4* pair_tab;
   const float* pe_ref;    // [Nc]

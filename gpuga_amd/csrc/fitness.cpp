@@ -82,8 +82,9 @@ private:
 
 class NN2BPotential : public Potential {
 public:
-  explicit NN2BPotential(int n) : n_(n) {}
-  void initialize(int, int) override { w_.resize(n_ * (n_ + 4) + 1); }
+  // species: the first layer also takes the pair's species code
+  NN2BPotential(int n, bool species) : n_(n), species_(species) {}
+  void initialize(int, int) override { w_.resize(n_ * (n_ + 4) + 1 + (species_ ? n_ : 0)); }
   void update_potential(const std::vector<float>& p) override
   {
     host_ = p;
@@ -96,11 +97,12 @@ public:
       w_.upload(host_, st);
       dirty_ = false;
     }
-    launch_nn2b_plain(s, n_, w_.get(), f, virial, pe, st);
+    launch_nn2b_plain(s, n_, species_, w_.get(), f, virial, pe, st);
   }
 
 private:
   int n_;
+  bool species_;
   std::vector<float> host_;
   bool dirty_ = false;
   DeviceBuffer<float> w_;
@@ -118,8 +120,9 @@ std::unique_ptr<Potential> make_potential(const PotentialSpec& spec, const Devic
     return p;
   }
   case POT_RI: return std::make_unique<RIPotential>(ds.ri_tab.get());
-  case POT_NN2B: {
-    auto p = std::make_unique<NN2BPotential>(spec.num_neurons);
+  case POT_NN2B:
+  case POT_NN2B_SPECIES: {
+    auto p = std::make_unique<NN2BPotential>(spec.num_neurons, spec.type == POT_NN2B_SPECIES);
     p->initialize(0, 0);
     return p;
   }
@@ -203,9 +206,11 @@ void Fitness::setup()
   GPUGA_HIP(hipEventCreate(&ev1_));
 
   const bool tersoff = spec_.type == POT_TERSOFF1 || spec_.type == POT_TERSOFF2;
-  if (spec_.type == POT_RI)
+  if (spec_.type == POT_RI || spec_.type == POT_NN2B_SPECIES)
     for (int t : ts_.type)
-      if (t != 0 && t != 1) fail("RI potential needs atom types 0 and 1");
+      if (t != 0 && t != 1)
+        fail(spec_.type == POT_RI ? "RI potential needs atom types 0 and 1"
+                                  : "NN2B with the species input needs atom types 0 and 1");
   nl_ = build_neighbors_device(ts_, spec_.cutoff, stream_);
   for (int i = 0; i < ts_.N; ++i)
     for (int s = 0; s < nl_.NN[i]; ++s)
@@ -390,7 +395,7 @@ void Fitness::setup()
     ds_.pair_tab.upload(pt, st);
     ds_.pair_hdr.upload(hdr, st);
   }
-  if (spec_.type == POT_NN2B) {
+  if (spec_.nn2b()) {
     // population form: per atom the pairs it evaluates (all bonds for force
     // configurations, j > i for the others) in order of distance, so that
     // consecutive pairs read neighbouring rows of the tabulated network; per
@@ -428,8 +433,11 @@ void Fitness::setup()
         *out++ = make_float4((float)(r.x * dinv), (float)(r.y * dinv), (float)(r.z * dinv), r.w);
         *out++ = make_float4((float)(2 * t3 - 3 * t2 + 1), (float)(h * (t3 - 2 * t2 + t)),
                              (float)(-2 * t3 + 3 * t2), (float)(h * (t3 - t2)));
+        // (the species code t_i + t_j picks the individual's table for it)
+        const int code = spec_.type == POT_NN2B_SPECIES ? ptype[b] : 0;
         float4 c = make_float4((float)fc, (float)fcp, 0.0f, 0.0f);
         std::memcpy(&c.z, &g, sizeof g);
+        std::memcpy(&c.w, &code, sizeof code);
         *out++ = c;
       }
     });
@@ -486,6 +494,7 @@ NN2BGrid Fitness::nn2b_grid_desc() const
   g.G = std::max(opt_.nn2b_grid, 16);
   g.n = spec_.num_neurons;
   g.bf16 = opt_.nn2b_bf16;
+  g.codes = spec_.pair_codes();
   // one spare cell on each side of the training set's pair-distance range
   const float span = std::max(dmax_ - dmin_, 1e-3f);
   g.h = span / (float)(g.G - 3);
@@ -546,7 +555,8 @@ size_t Fitness::plan_bytes(int P) const
   size_t b = K * 3 * ts_.Nc * WAVE * sizeof(double);              // err
   b += (size_t)2 * ts_.Nc * K * 8 * WAVE * sizeof(double);         // part (typical)
   if (tersoff) b += K * (B_force + ts_.N_force) * 3 * WAVE * sizeof(float);       // bondf
-  if (spec_.type == POT_NN2B) b += (size_t)3 * opt_.nn2b_grid * K * WAVE * sizeof(float);
+  if (spec_.nn2b())
+    b += (size_t)3 * spec_.pair_codes() * opt_.nn2b_grid * K * WAVE * sizeof(float);
   return b;
 }
 
@@ -671,7 +681,8 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
     if (occ <= 0) {
       switch (spec_.type) {
       case POT_RI: occ = ri_population_occupancy(pl.block); break;
-      case POT_NN2B: occ = nn2b_population_occupancy(pl.block); break;
+      case POT_NN2B:
+      case POT_NN2B_SPECIES: occ = nn2b_population_occupancy(pl.block); break;
       default:
         occ = loop_path_ ? tersoff_loop_occupancy(spec_.type == POT_TERSOFF2, pn.max_nn, pl.block)
                          : tersoff_population_occupancy(nn_class(pn.max_nn),
@@ -759,7 +770,8 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   const int B_force =
     ts_.N_force > 0 ? pn.start[ts_.N_force - 1] + pn.NN[ts_.N_force - 1] : 0;
   pl.bondf.resize(tersoff ? std::max<size_t>((size_t)K * (B_force + ts_.N_force) * 3 * WAVE, 1) : 1);
-  if (spec_.type == POT_NN2B) pl.nn2b_table.resize((size_t)K * nn2b_grid_desc().G * 3 * WAVE);
+  if (spec_.nn2b())
+    pl.nn2b_table.resize((size_t)spec_.pair_codes() * K * nn2b_grid_desc().G * 3 * WAVE);
   const int groups = (Nc + CFG_GROUP - 1) / CFG_GROUP;
   pl.gpart.resize((size_t)K * groups * 3 * WAVE);
   // counters: cnt [Nc*K] | cnt2 [Nc*K] | grp_done [K*groups] | blk_done [K] |
@@ -834,7 +846,8 @@ void Fitness::launch_population(PopPlan& pl, hipStream_t st, const float* genes)
   if (genes) ctl.genes = genes;
   switch (spec_.type) {
   case POT_RI: launch_ri_population(ctl, ds_.view, ds_.ri_tab.get(), pl.grid, pl.block, st); break;
-  case POT_NN2B: {
+  case POT_NN2B:
+  case POT_NN2B_SPECIES: {
     NN2BGrid g = nn2b_grid_desc();
     launch_nn2b_tabulate(ctl, g, pl.nn2b_table.get(), st);
     launch_nn2b_population(ctl, ds_.view, g, pl.nn2b_table.get(), pl.grid, pl.block, st);
@@ -1256,7 +1269,7 @@ size_t Fitness::estimate_bytes(const PotentialSpec& s, const TrainSet& ts, int P
   const bool tersoff = s.type == POT_TERSOFF1 || s.type == POT_TERSOFF2;
   size_t b = N * sizeof(int) * 5 + Nc * sizeof(int) * 2 + B * (3 * sizeof(int) + 16);
   b += (tersoff ? tri * 16 + B * 4 * 5 : 0) + (s.type == POT_RI ? B * 48 + N * 16 : 0);
-  b += s.type == POT_NN2B ? B * 48 + N * 16 : 0;
+  b += s.nn2b() ? B * 48 + N * 16 : 0;
   b += Nc * 7 * sizeof(float) + N * 3 * sizeof(float);  // references
   b += N * 10 * sizeof(float) + Nc * 3 * sizeof(double); // per-atom outputs
   const size_t K = (P + WAVE - 1) / WAVE;
@@ -1264,7 +1277,7 @@ size_t Fitness::estimate_bytes(const PotentialSpec& s, const TrainSet& ts, int P
   for (int i = 0; i < ts.N_force; ++i) B_force += nn[i];
   b += K * 3 * Nc * WAVE * sizeof(double) + 2 * Nc * K * 8 * WAVE * sizeof(double);
   if (tersoff) b += K * (B_force + ts.N_force) * 3 * WAVE * sizeof(float);
-  if (s.type == POT_NN2B) b += (size_t)3 * opt.nn2b_grid * K * WAVE * sizeof(float);
+  if (s.nn2b()) b += (size_t)3 * s.pair_codes() * opt.nn2b_grid * K * WAVE * sizeof(float);
   b += (size_t)P * s.V * sizeof(float) + (size_t)P * 4 * sizeof(double);
   return b;
 }

@@ -13,9 +13,10 @@ void launch_tersoff_plain(const DevSet& s, const TersoffParams& q, float* b, flo
 // tab: per-bond RI factors (r^-6, erfc(a r)/r, -(d/dr) erfc(a r)/r, 1/r)
 void launch_ri_plain(const DevSet& s, const RIParams& q, const float4* tab, float* f,
                      float* virial, float* pe, hipStream_t st);
-// w: device pointer to the n(n+4)+1 physical parameters
-void launch_nn2b_plain(const DevSet& s, int n, const float* w, float* f, float* virial,
-                       float* pe, hipStream_t st);
+// w: device pointer to the n(n+4)+1 physical parameters (species: then the
+// n weights of the species-pair input)
+void launch_nn2b_plain(const DevSet& s, int n, bool species, const float* w, float* f,
+                       float* virial, float* pe, hipStream_t st);
 
 // ---- fitness of one parameter set from per-atom outputs
 // cerr: [Nc][3] scratch; out: fitness (float) and terms[3] (double)
@@ -50,6 +51,7 @@ struct NN2BGrid {
   float d0, h;    // first point and spacing
   int n;          // neurons
   int bf16;       // 1: hidden layer 2 on BF16 MFMA, 0: FP32 VALU
+  int codes;      // species codes tabulated (1; 3 with the species-pair input)
 };
 void launch_nn2b_tabulate(const PopCtl& ctl, const NN2BGrid& g, float* table, hipStream_t st);
 void launch_nn2b_population(const PopCtl& ctl, const DevSet& s, const NN2BGrid& g,

@@ -17,10 +17,11 @@ namespace gpuga {
 
 namespace {
 
-// table layout: [K][G][3][64] (p, p', p'' for the 64 individuals of block k)
-__device__ inline size_t tab_index(int k, int G, int g, int q, int lane)
+// table layout: [code][K][G][3][64] (p, p', p'' for the 64 individuals of
+// block k, per species code of the pair; one code without the species input)
+__device__ inline size_t tab_index(int code, int K, int k, int G, int g, int q, int lane)
 {
-  return (((size_t)k * G + g) * 3 + q) * WAVE + lane;
+  return ((((size_t)code * K + k) * G + g) * 3 + q) * WAVE + lane;
 }
 
 // FP32 tabulation: block per individual (weights staged in LDS), thread per
@@ -30,7 +31,7 @@ __global__ void __launch_bounds__(256) nn2b_tabulate_f32(PopCtl ctl, NN2BGrid gr
                                                          float* __restrict__ table)
 {
   extern __shared__ float w[];
-  const int p = blockIdx.y;
+  const int p = blockIdx.y, code = blockIdx.z;
   const int n = grid.n, V = ctl.V;
   for (int v = threadIdx.x; v < V; v += blockDim.x)
     w[v] = denormalise(ctl.pmin[v], ctl.pmax[v], ctl.genes[(size_t)p * V + v]);
@@ -43,12 +44,15 @@ __global__ void __launch_bounds__(256) nn2b_tabulate_f32(PopCtl ctl, NN2BGrid gr
   const float* b1 = w + n * (n + 2);
   const float* w2 = w + n * (n + 3);
   const float b2 = w[n * (n + 4)];
+  const float* ws = w + n * (n + 4) + 1; // species-pair weights (codes > 1)
   const float d = grid.d0 + grid.h * g;
   float x1[NMAX], y1[NMAX], y2[NMAX];
 #pragma unroll
   for (int m = 0; m < NMAX; ++m) {
     if (m < n) {
-      const float t = tanhf(w0[m] * d - b0[m]);
+      // the species-pair input as a shift of the bias (none for one code)
+      const float bm = grid.codes > 1 ? b0[m] - ws[m] * (float)code : b0[m];
+      const float t = tanhf(w0[m] * d - bm);
       const float s = 1.0f - t * t;
       x1[m] = t;
       y1[m] = s * w0[m];                      // d x1 / dr
@@ -74,9 +78,9 @@ __global__ void __launch_bounds__(256) nn2b_tabulate_f32(PopCtl ctl, NN2BGrid gr
     pdd += w2[a] * (s * zdd - 2.0f * x2 * s * zd * zd);
   }
   const int k = p / WAVE, lane = p % WAVE;
-  table[tab_index(k, grid.G, g, 0, lane)] = pv - b2;
-  table[tab_index(k, grid.G, g, 1, lane)] = pd;
-  table[tab_index(k, grid.G, g, 2, lane)] = pdd;
+  table[tab_index(code, ctl.K, k, grid.G, g, 0, lane)] = pv - b2;
+  table[tab_index(code, ctl.K, k, grid.G, g, 1, lane)] = pd;
+  table[tab_index(code, ctl.K, k, grid.G, g, 2, lane)] = pdd;
 }
 
 } // namespace
@@ -91,6 +95,7 @@ struct NN2BPop {
   struct Extra {
     NN2BGrid g;
     const float* table;
+    long code_stride; // floats from one species code's tables to the next
   };
 
   __device__ static Lane make(const PopCtl&, int) { return Lane{0}; }
@@ -103,9 +108,11 @@ struct NN2BPop {
   struct Rows {
     float p0, d0, s0, p1, d1, s1;
   };
-  __device__ static __forceinline__ Rows rows(const float* __restrict__ tk, const float4& c)
+  __device__ static __forceinline__ Rows rows(const float* __restrict__ tk, long code_stride,
+                                              const float4& c)
   {
-    const float* r0 = tk + (size_t)__float_as_int(c.z) * 3 * WAVE;
+    const float* r0 = tk + (long)__float_as_int(c.w) * code_stride +
+                      (size_t)__float_as_int(c.z) * 3 * WAVE;
     return Rows{r0[0], r0[WAVE], r0[2 * WAVE], r0[3 * WAVE], r0[4 * WAVE], r0[5 * WAVE]};
   }
   __device__ static __forceinline__ void pair(const Rows& r, const float4& w, const float4& c,
@@ -121,14 +128,15 @@ struct NN2BPop {
   // lookups (a gather that depends on the scalar load of the cell) are
   // waited for once per step rather than once per pair.
   template <class F>
-  __device__ static __forceinline__ void each_pair(const float* __restrict__ tk,
+  __device__ static __forceinline__ void each_pair(const float* __restrict__ tk, long cs,
                                                    const float4* tab, int n, F&& f)
   {
     int k = 0;
     for (; k + 4 <= n; k += 4) {
       const float4 c0 = ldu(tab, 3 * k + 2), c1 = ldu(tab, 3 * k + 5);
       const float4 c2 = ldu(tab, 3 * k + 8), c3 = ldu(tab, 3 * k + 11);
-      const Rows r0 = rows(tk, c0), r1 = rows(tk, c1), r2 = rows(tk, c2), r3 = rows(tk, c3);
+      const Rows r0 = rows(tk, cs, c0), r1 = rows(tk, cs, c1), r2 = rows(tk, cs, c2),
+                 r3 = rows(tk, cs, c3);
       f(ldu(tab, 3 * k), r0, ldu(tab, 3 * k + 1), c0);
       f(ldu(tab, 3 * k + 3), r1, ldu(tab, 3 * k + 4), c1);
       f(ldu(tab, 3 * k + 6), r2, ldu(tab, 3 * k + 7), c2);
@@ -137,13 +145,13 @@ struct NN2BPop {
     for (; k + 2 <= n; k += 2) {
       const float4 u0 = ldu(tab, 3 * k), w0 = ldu(tab, 3 * k + 1), c0 = ldu(tab, 3 * k + 2);
       const float4 u1 = ldu(tab, 3 * k + 3), w1 = ldu(tab, 3 * k + 4), c1 = ldu(tab, 3 * k + 5);
-      const Rows r0 = rows(tk, c0), r1 = rows(tk, c1);
+      const Rows r0 = rows(tk, cs, c0), r1 = rows(tk, cs, c1);
       f(u0, r0, w0, c0);
       f(u1, r1, w1, c1);
     }
     if (k < n) {
       const float4 u = ldu(tab, 3 * k), w = ldu(tab, 3 * k + 1), c = ldu(tab, 3 * k + 2);
-      f(u, rows(tk, c), w, c);
+      f(u, rows(tk, cs, c), w, c);
     }
   }
 
@@ -158,8 +166,8 @@ struct NN2BPop {
         const int4 h = ldu(s.pair_hdr, i);
         const float4* tab = s.pair_tab + 3 * (long)h.x;
         float fx = 0, fy = 0, fz = 0;
-        each_pair(tk, tab, h.y, [&](const float4& u, const Rows& r, const float4& w,
-                                    const float4& c) {
+        each_pair(tk, ex.code_stride, tab, h.y,
+                  [&](const float4& u, const Rows& r, const float4& w, const float4& c) {
           float p2, G;
           pair(r, w, c, p2, G);
           fx = fmaf(u.x, G, fx);
@@ -176,8 +184,8 @@ struct NN2BPop {
         const int4 h = ldu(s.pair_hdr, i);
         const float4* tab = s.pair_tab + 3 * (long)h.x;
         float e = 0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wyz = 0, wzx = 0;
-        each_pair(tk, tab, h.y, [&](const float4& u, const Rows& r, const float4& w,
-                                    const float4& c) {
+        each_pair(tk, ex.code_stride, tab, h.y,
+                  [&](const float4& u, const Rows& r, const float4& w, const float4& c) {
           float p2, G;
           pair(r, w, c, p2, G);
           const float t = u.w * G;
@@ -209,7 +217,7 @@ struct NN2BPop {
 
 void launch_nn2b_tabulate_f32(const PopCtl& ctl, const NN2BGrid& g, float* table, hipStream_t st)
 {
-  const dim3 block(256), grid((g.G + 255) / 256, ctl.P);
+  const dim3 block(256), grid((g.G + 255) / 256, ctl.P, g.codes);
   const size_t lds = sizeof(float) * ctl.V;
   if (g.n <= 8)
     nn2b_tabulate_f32<8><<<grid, block, lds, st>>>(ctl, g, table);
@@ -237,7 +245,7 @@ void launch_nn2b_tabulate(const PopCtl& ctl, const NN2BGrid& g, float* table, hi
 void launch_nn2b_population(const PopCtl& ctl, const DevSet& s, const NN2BGrid& g,
                             const float* table, int grid, int block, hipStream_t st)
 {
-  NN2BPop::Extra ex{g, table};
+  NN2BPop::Extra ex{g, table, (long)ctl.K * g.G * 3 * WAVE};
   population_kernel<NN2BPop><<<grid, block, 0, st>>>(ctl, s, ex);
   GPUGA_HIP(hipGetLastError());
 }

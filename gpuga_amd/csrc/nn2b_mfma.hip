@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(64) nn2b_tabulate_bf16(PopCtl ctl, NN2BGrid gr
                                                          float* __restrict__ table)
 {
   __shared__ float red[3][TILE][TILE + 1];
-  const int p = blockIdx.y;
+  const int p = blockIdx.y, code = blockIdx.z;
   const int l = threadIdx.x, col = l & 31, kg = l >> 5;
   const int n = grid.n, V = ctl.V;
   const float* gp = ctl.genes + (size_t)p * V;
@@ -64,7 +64,9 @@ __global__ void __launch_bounds__(64) nn2b_tabulate_bf16(PopCtl ctl, NN2BGrid gr
       const int m = 16 * s + 8 * kg + j;
       const bool ok = m < n;
       w0m[8 * s + j] = ok ? W(m) : 0.0f;
-      b0m[8 * s + j] = ok ? W(n + m) : 0.0f;
+      // the species-pair input (codes > 1) enters as a shift of the bias
+      b0m[8 * s + j] =
+        ok ? W(n + m) - (grid.codes > 1 ? W(n * (n + 4) + 1 + m) * (float)code : 0.0f) : 0.0f;
       split_bf16(ok && col < n ? W(2 * n + col * n + m) : 0.0f, Bh[s], Bl[s], j);
     }
   }
@@ -111,7 +113,8 @@ __global__ void __launch_bounds__(64) nn2b_tabulate_bf16(PopCtl ctl, NN2BGrid gr
       for (int c = 0; c < TILE; ++c) sum += red[q][row][c];
       if (q == 0) sum -= b2;
       const int g = g0 + row;
-      if (g < grid.G) table[(((size_t)k * grid.G + g) * 3 + q) * WAVE + lp] = sum;
+      if (g < grid.G)
+        table[((((size_t)code * ctl.K + k) * grid.G + g) * 3 + q) * WAVE + lp] = sum;
     }
     __syncthreads();
   }
@@ -129,7 +132,7 @@ void launch_nn2b_tabulate_bf16(const PopCtl& ctl, const NN2BGrid& g, float* tabl
   }
   const int ntiles = (g.G + TILE - 1) / TILE;
   const int tiles = 4;
-  const dim3 grid((ntiles + tiles - 1) / tiles, ctl.P), block(64);
+  const dim3 grid((ntiles + tiles - 1) / tiles, ctl.P, g.codes), block(64);
   nn2b_tabulate_bf16<<<grid, block, 0, st>>>(ctl, g, tiles, table);
   GPUGA_HIP(hipGetLastError());
 }

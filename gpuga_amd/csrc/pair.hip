@@ -268,12 +268,15 @@ int ri_population_occupancy(int block)
 constexpr int NN2B_MAX_PLAIN = 64;
 
 // One thread per atom, weights staged in LDS; the MLP is evaluated per bond as
-// the reference does (src/nn2b.cu:49-78), for any n <= 64.
-__global__ void nn2b_plain(DevSet s, int n, const float* __restrict__ w, float* __restrict__ f,
-                           float* __restrict__ virial, float* __restrict__ pe)
+// the reference does (src/nn2b.cu:49-78), for any n <= 64.  species: the
+// first layer also takes the pair's species code t_i + t_j through the n
+// weights after the reference's parameters.
+__global__ void nn2b_plain(DevSet s, int n, int species, const float* __restrict__ w,
+                           float* __restrict__ f, float* __restrict__ virial,
+                           float* __restrict__ pe)
 {
   extern __shared__ float sw[];
-  const int V = n * (n + 4) + 1;
+  const int V = n * (n + 4) + 1 + (species ? n : 0);
   for (int v = threadIdx.x; v < V; v += blockDim.x) sw[v] = w[v];
   __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -284,6 +287,7 @@ __global__ void nn2b_plain(DevSet s, int n, const float* __restrict__ w, float* 
   const float* b1 = sw + n * (n + 2);
   const float* w2 = sw + n * (n + 3);
   const float bias2 = sw[n * (n + 4)];
+  const float* ws = sw + n * (n + 4) + 1;
   const size_t N = s.N;
   const int nn = s.nn[i], k0 = s.bond0[i];
   float fx = 0, fy = 0, fz = 0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wyz = 0, wzx = 0, e = 0;
@@ -291,8 +295,11 @@ __global__ void nn2b_plain(DevSet s, int n, const float* __restrict__ w, float* 
   for (int u = 0; u < nn; ++u) {
     const float4 r = s.bond[k0 + u];
     const float d = r.w;
+    const float code = species ? (float)s.ptype[k0 + u] : 0.0f;
     for (int m = 0; m < n; ++m) {
-      x1[m] = tanhf(w0[m] * d - b0[m]);
+      // the species-pair input as a shift of the bias
+      const float bm = species ? b0[m] - ws[m] * code : b0[m];
+      x1[m] = tanhf(w0[m] * d - bm);
       y1[m] = (1.0f - x1[m] * x1[m]) * w0[m];
     }
     float p = 0.0f, g = 0.0f;
@@ -346,13 +353,13 @@ __global__ void nn2b_plain(DevSet s, int n, const float* __restrict__ w, float* 
   pe[i] = e;
 }
 
-void launch_nn2b_plain(const DevSet& s, int n, const float* w, float* f, float* virial,
-                       float* pe, hipStream_t st)
+void launch_nn2b_plain(const DevSet& s, int n, bool species, const float* w, float* f,
+                       float* virial, float* pe, hipStream_t st)
 {
   if (n > NN2B_MAX_PLAIN) fail(strfmt("NN2B supports at most %d neurons", NN2B_MAX_PLAIN));
   const int block = 128, grid = (s.N + block - 1) / block;
-  const size_t lds = sizeof(float) * (n * (n + 4) + 1);
-  nn2b_plain<<<grid, block, lds, st>>>(s, n, w, f, virial, pe);
+  const size_t lds = sizeof(float) * (n * (n + 4) + 1 + (species ? n : 0));
+  nn2b_plain<<<grid, block, lds, st>>>(s, n, species ? 1 : 0, w, f, virial, pe);
   GPUGA_HIP(hipGetLastError());
 }
 

@@ -187,9 +187,11 @@ PotentialSpec read_potential(const std::string& dir)
   case POT_TERSOFF1: s.V = 9; break;
   case POT_TERSOFF2: s.V = 13; break;
   case POT_NN2B:
+  case POT_NN2B_SPECIES:
     if (!in.get_word(name) || !in.get_int(s.num_neurons)) fail(bad);
     if (s.num_neurons < 1) fail("num_neurons should >= 1.");
-    s.V = s.num_neurons * (s.num_neurons + 4) + 1;
+    s.V = s.num_neurons * (s.num_neurons + 4) + 1 +
+          (s.type == POT_NN2B_SPECIES ? s.num_neurons : 0);
     break;
   default: fail("unsupported potential type.");
   }

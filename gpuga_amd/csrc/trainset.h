@@ -17,7 +17,17 @@
 
 namespace gpuga {
 
-enum PotentialType { POT_RI = 0, POT_TERSOFF1 = 1, POT_TERSOFF2 = 2, POT_NN2B = 3 };
+// POT_NN2B_SPECIES (no reference counterpart; the reference rejects type 4):
+// NN2B whose first layer also takes the pair's species code s = t_i + t_j,
+// x1 = tanh(w0 r + ws s - b0), with the n weights ws after the reference's
+// n (n + 4) + 1 parameters.  Types must be 0 and 1.
+enum PotentialType {
+  POT_RI = 0,
+  POT_TERSOFF1 = 1,
+  POT_TERSOFF2 = 2,
+  POT_NN2B = 3,
+  POT_NN2B_SPECIES = 4
+};
 
 struct PotentialSpec {
   int type = POT_TERSOFF1;
@@ -27,6 +37,9 @@ struct PotentialSpec {
   float w_force = 0.0f, w_energy = 0.0f, w_stress = 0.0f;
   std::vector<float> pmin, pmax;
   std::vector<std::string> names;
+  bool nn2b() const { return type == POT_NN2B || type == POT_NN2B_SPECIES; }
+  // species codes a pair can have (NN2B with the species input: 0, 1, 2)
+  int pair_codes() const { return type == POT_NN2B_SPECIES ? 3 : 1; }
 };
 
 struct TrainSet {

@@ -14,6 +14,7 @@ Workloads (sizes as named in the benchmark configurations):
   fluorite_ab2(...) AB2 fluorite 2x2x2 cells (96 atoms), a = 5.1 A, +-2%
                     strain, N(0, 0.05 A); rigid-ion potential, 15 parameters
   wurtzite_ab(...)  AB wurtzite 3x3x2 cells (72 atoms); NN2B pair potential
+                    (species=True: with the species pair as a network input)
 """
 from __future__ import annotations
 
@@ -149,7 +150,9 @@ def nn2b_truth(n, seed=7):
 
 
 def wurtzite_ab(Nc=800, a=3.82, c_over_a=1.633, u=0.375, strain=0.02, disp=0.05, n=32,
-                force_fraction=0.2, seed=5):
+                force_fraction=0.2, seed=5, species=False):
+    """AB wurtzite cells for NN2B; species=True: potential type 4, whose first
+    layer also takes the pair's species code t_i + t_j (n more weights)."""
     rng = np.random.default_rng(seed)
     frac = np.array([[1 / 3, 2 / 3, 0], [2 / 3, 1 / 3, .5], [1 / 3, 2 / 3, u], [2 / 3, 1 / 3, .5 + u]])
     t = np.array([0, 0, 1, 1], np.int32)
@@ -162,10 +165,17 @@ def wurtzite_ab(Nc=800, a=3.82, c_over_a=1.633, u=0.375, strain=0.02, disp=0.05,
         pos = f2 @ A + rng.normal(0.0, disp, f2.shape)
         cells.append((A, pos, t2))
     d = _assemble(cells, force_fraction, rng)
-    V = n * (n + 4) + 1
-    d.update(potential_type=3, num_neurons=n, cutoff=6.0, weights=(0.9, 0.1, 0.0),
+    V = n * (n + 4) + 1 + (n if species else 0)
+    truth = nn2b_truth(n)
+    if species:
+        # species weights: A-A, A-B, B-B pairs see first-layer inputs shifted by
+        # 0, ws, 2 ws
+        ws = np.random.default_rng(seed + 1000).uniform(-0.5, 0.5, n).astype(np.float32)
+        truth = np.concatenate([truth, ws]).astype(np.float32)
+    d.update(potential_type=4 if species else 3, num_neurons=n, cutoff=6.0,
+             weights=(0.9, 0.1, 0.0),
              pmin=np.full(V, -3, np.float32), pmax=np.full(V, 3, np.float32),
-             names=[f"w{k}" for k in range(V)], truth=nn2b_truth(n))
+             names=[f"w{k}" for k in range(V)], truth=truth)
     return d
 
 
@@ -198,7 +208,7 @@ def write_dir(d, path, ga=None):
     names = d.get("names")
     with open(os.path.join(path, "potential.in"), "w") as f:
         f.write(f"potential_type {d['potential_type']}\n")
-        if d["potential_type"] == 3:
+        if d["potential_type"] in (3, 4):
             f.write(f"num_neurons {d['num_neurons']}\n")
         wf, we, wv = d["weights"]
         f.write(f"cutoff {d['cutoff']!r}\nweight_force {wf!r}\nweight_energy {we!r}\n"

@@ -50,11 +50,13 @@ int orc_read_potential(const char* dir, orc_potential* pot)
   case 1: pot->num_variables = 9; break;
   case 2: pot->num_variables = 13; break;
   case 3:
+  case 4: /* 4: the species-pair input, n more weights */
     if (fscanf(fid, "%255s%d", name, &pot->num_neurons) != 2) {
       rc = fail("reading error for potential.in.");
       goto done;
     }
-    pot->num_variables = pot->num_neurons * (pot->num_neurons + 4) + 1;
+    pot->num_variables = pot->num_neurons * (pot->num_neurons + 4) + 1 +
+                         (pot->potential_type == 4 ? pot->num_neurons : 0);
     break;
   default: rc = fail("unsupported potential type."); goto done;
   }

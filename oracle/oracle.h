@@ -15,6 +15,7 @@
  *   - minimal Tersoff 1/2    src/minimal_tersoff.cu:39-405   (FP64)
  *   - RI                     src/ri.cu:36-187                (FP64)
  *   - NN2B                   src/nn2b.cu:33-171              (FP64)
+ *   - NN2B + species input   type 4, no reference counterpart (FP64; see pair.c)
  *   - fitness reduction      src/fitness.cu:316-333,437-555  (FP64)
  *   - genetic algorithm      src/ga.cu:27-212 with libstdc++ <random>
  *                            semantics (mt19937, generate_canonical<float>,
@@ -37,7 +38,8 @@ extern "C" {
 /* ---------------------------------------------------------------- inputs */
 
 typedef struct {
-  int potential_type;  /* 0 RI, 1 Tersoff-1, 2 Tersoff-2, 3 NN2B */
+  int potential_type;  /* 0 RI, 1 Tersoff-1, 2 Tersoff-2, 3 NN2B, 4 NN2B with the
+                          species-pair input */
   int num_neurons;     /* NN2B only */
   int num_variables;
   float cutoff;        /* neighbour-list cutoff */
@@ -101,7 +103,7 @@ void orc_tersoff(int num_types, const float* params, const orc_trainset* ts,
                  const orc_neighbor* nb, double* f, double* virial, double* pe);
 void orc_ri(const float* params, const orc_trainset* ts, const orc_neighbor* nb,
             double* f, double* virial, double* pe);
-void orc_nn2b(int num_neurons, const float* params, const orc_trainset* ts,
+void orc_nn2b(int num_neurons, int species, const float* params, const orc_trainset* ts,
               const orc_neighbor* nb, double* f, double* virial, double* pe);
 int orc_find_force(const orc_potential* pot, const float* params, const orc_trainset* ts,
                    const orc_neighbor* nb, double* f, double* virial, double* pe);

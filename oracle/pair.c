@@ -6,6 +6,11 @@
  *         :80-92 (cutoff), :94-171 (pair loop, including f2 = (p'fc + U fc')/r
  *         with U = p fc, nn2b.cu:143-148), src/nn2b.cuh:36-38 (r1 = 7,
  *         r2 = 8, pi_factor = 3.1415927f, scaling = 1)
+ *   NN2B with a species-pair input (potential type 4; no counterpart in the
+ *         reference): the same network and pair loop, with the first layer
+ *         x1 = tanh(w0 r + ws s - b0), s = t_i + t_j, and the n weights ws
+ *         after the reference's n (n + 4) + 1 parameters.  Pinned by: ws = 0
+ *         gives type 3 exactly; finite differences; a dimer closed form.
  */
 #include "oracle.h"
 
@@ -71,16 +76,19 @@ typedef struct {
   const float* w1;
   const float* b1;
   const float* w2;
+  const float* ws; /* species-pair weights (type 4), else NULL */
   double b2;
   double* x1;
   double* y1;
 } nnpar;
 
-static void nn_p2_f2(const nnpar* q, double d, double* p2, double* f2)
+/* s: the pair's species code t_i + t_j (used when q->ws is set) */
+static void nn_p2_f2(const nnpar* q, double d, int s, double* p2, double* f2)
 {
   const int n = q->n;
   for (int m = 0; m < n; ++m) {
-    q->x1[m] = tanh((double)q->w0[m] * d - (double)q->b0[m]);
+    const double in = q->ws ? (double)q->ws[m] * s : 0.0;
+    q->x1[m] = tanh((double)q->w0[m] * d + in - (double)q->b0[m]);
     q->y1[m] = (1.0 - q->x1[m] * q->x1[m]) * (double)q->w0[m];
   }
   double p = 0.0, g = 0.0;
@@ -137,7 +145,7 @@ static void pair_loop(int kind, const void* par, const orc_trainset* ts, const o
       if (kind == 0) {
         ri_p2_f2((const ripar*)par, ts->type[i] + ts->type[j], d, &p2, &f2);
       } else {
-        nn_p2_f2((const nnpar*)par, d, &p2, &f2);
+        nn_p2_f2((const nnpar*)par, d, ts->type[i] + ts->type[j], &p2, &f2);
         double fc, fcp;
         if (d < nn_r1) {
           fc = 1.0;
@@ -177,11 +185,12 @@ void orc_ri(const float* params, const orc_trainset* ts, const orc_neighbor* nb,
   pair_loop(0, &q, ts, nb, f, virial, pe);
 }
 
-void orc_nn2b(int n, const float* params, const orc_trainset* ts, const orc_neighbor* nb,
-              double* f, double* virial, double* pe)
+void orc_nn2b(int n, int species, const float* params, const orc_trainset* ts,
+              const orc_neighbor* nb, double* f, double* virial, double* pe)
 {
   nnpar q;
   q.n = n;
+  q.ws = species ? params + n * (n + 4) + 1 : NULL;
   q.w0 = params;
   q.b0 = params + n;
   q.w1 = params + 2 * n;
@@ -202,7 +211,8 @@ int orc_find_force(const orc_potential* pot, const float* params, const orc_trai
   case 0: orc_ri(params, ts, nb, f, virial, pe); return 0;
   case 1: orc_tersoff(1, params, ts, nb, f, virial, pe); return 0;
   case 2: orc_tersoff(2, params, ts, nb, f, virial, pe); return 0;
-  case 3: orc_nn2b(pot->num_neurons, params, ts, nb, f, virial, pe); return 0;
+  case 3: orc_nn2b(pot->num_neurons, 0, params, ts, nb, f, virial, pe); return 0;
+  case 4: orc_nn2b(pot->num_neurons, 1, params, ts, nb, f, virial, pe); return 0;
   }
   return 1;
 }

@@ -64,3 +64,64 @@ def test_tabulated_forces_energies_virials_match_oracle(bf16, tol):
     assert np.all(terms[1] < tol)
     # the other individuals are far from these labels
     assert np.all(terms[[0, 2]] > 10 * terms[1].max())
+
+
+def _species_set(Nc, n, seed, ws_zero=False):
+    from gpuga_amd import synth
+    from oracle import oracle as O
+    d = synth.wurtzite_ab(Nc=Nc, n=n, seed=seed, species=True)
+    if ws_zero:
+        d["truth"] = d["truth"].copy()
+        d["truth"][n * (n + 4) + 1:] = 0.0
+    pot = dict(potential_type=4, cutoff=d["cutoff"], weights=d["weights"], pmin=d["pmin"],
+               pmax=d["pmax"], num_neurons=n)
+    d = synth.label(d, O.Case(potential=pot, arrays=d).find_force)
+    return d, pot
+
+
+@pytest.mark.parametrize("bf16,tol", [(0, 1e-5), (1, 1e-4)])
+def test_species_pair_input_matches_oracle(bf16, tol):
+    """NN2B with the species-pair input (potential type 4): the first layer
+    sees the pair's species code t_i + t_j; the population path tabulates the
+    network once per code.  Fitness and terms of individuals of two lane
+    blocks against the oracle; per-atom outputs of the per-individual path
+    too."""
+    import gpuga_amd as G
+    from oracle import oracle as O
+    print_case("nn2b species-pair input", bf16)
+    n = 10
+    d, pot = _species_set(40, n, 16)
+    case = O.Case(potential=pot, arrays=d)
+    genes = np.random.default_rng(17).random((70, case.V), dtype=np.float32)
+    with G.Fitness(arrays=d, nn2b_bf16=bf16) as fit:
+        assert fit.V == n * (n + 5) + 1
+        f, terms = fit.compute(genes, terms=True)
+        params = fit.denormalise(genes[5])
+        ff, fw, fe = fit.find_force(params)
+    sel = [0, 5, 63, 64, 69]
+    ref, ref_terms = case.fitness(genes[sel], threads=16)
+    np.testing.assert_allclose(terms[sel], ref_terms, rtol=tol)
+    np.testing.assert_allclose(f[sel], ref, rtol=tol)
+    of, ow, oe = case.find_force(params)
+    for got, want in ((ff, of), (fw, ow), (fe, oe)):
+        np.testing.assert_allclose(got, want, atol=2e-5 * np.abs(want).max())
+
+
+def test_species_pair_input_with_zero_weights_is_the_reference_form():
+    """Species weights of zero: type 4 gives type 3's fitness bit for bit on
+    the GPU (the same network, every code's table the same)."""
+    import gpuga_amd as G
+    print_case("nn2b species weights zero")
+    n = 10
+    d4, _ = _species_set(24, n, 18, ws_zero=True)
+    genes4 = np.random.default_rng(19).random((20, n * (n + 5) + 1), dtype=np.float32)
+    # genes of 0.5 are weights of 0 in [-3, 3]
+    genes4[:, n * (n + 4) + 1:] = 0.5
+    d3 = dict(d4)
+    d3.update(potential_type=3, pmin=d4["pmin"][: n * (n + 4) + 1],
+              pmax=d4["pmax"][: n * (n + 4) + 1])
+    out = []
+    for d, g in ((d3, genes4[:, : n * (n + 4) + 1]), (d4, genes4)):
+        with G.Fitness(arrays=d, nn2b_bf16=0) as fit:
+            out.append(fit.compute(g, terms=True))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])

@@ -45,7 +45,7 @@ def make_case(ptype, positions, types, box=32.0, cutoff=3.2, n=0, cells=None):
         r = np.concatenate([r, r], 1)
         typ = np.concatenate([typ, typ])
         boxes = np.concatenate([boxes, boxes])
-    V = {0: 15, 1: 9, 2: 13, 3: n * (n + 4) + 1}[ptype]
+    V = {0: 15, 1: 9, 2: 13, 3: n * (n + 4) + 1, 4: n * (n + 5) + 1}[ptype]
     pot = dict(potential_type=ptype, cutoff=cutoff, weights=(1.0, 1.0, 1.0),
                pmin=np.zeros(V, np.float32), pmax=np.ones(V, np.float32), num_neurons=n)
     arrays = dict(Na=Na, type=typ, box=boxes, r=r.reshape(-1), Nc_force=1,
@@ -277,3 +277,62 @@ def test_nn2b_taper_force_quirk():
     assert f[0] == pytest.approx(d * f2, rel=1e-10)
     assert abs(d * f2 - d * (dp * fc + p * fcp) / d) > 1e-6  # differs from the gradient
     del N
+
+
+# NN2B with the species-pair input (potential type 4, no reference
+# counterpart): x1 = tanh(w0 r + ws s - b0), s = t_i + t_j, ws after the
+# reference's n (n + 4) + 1 parameters.
+
+def _nn2b_species_params(n, seed):
+    rng = np.random.default_rng(seed)
+    w = rng.uniform(-1, 1, n * (n + 5) + 1).astype(np.float32)
+    w[:n] = rng.uniform(0.5, 1.5, n)
+    return w
+
+
+def test_nn2b_species_with_zero_weights_is_the_reference_form():
+    """ws = 0: type 4 gives type 3's forces, virials and energies bit for bit
+    on a binary wurtzite set."""
+    from gpuga_amd import synth
+    n = 6
+    d = synth.wurtzite_ab(Nc=3, n=n, seed=9)
+    w3 = _nn2b_species_params(n, 4)[: n * (n + 4) + 1]
+    w4 = np.concatenate([w3, np.zeros(n, np.float32)])
+    out = []
+    for ptype, w in ((3, w3), (4, w4)):
+        pot = dict(potential_type=ptype, cutoff=d["cutoff"], weights=d["weights"],
+                   pmin=np.zeros(len(w), np.float32), pmax=np.ones(len(w), np.float32),
+                   num_neurons=n)
+        out.append(O.Case(potential=pot, arrays=d).find_force(w))
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("types,code", [([0, 0], 0), ([0, 1], 1), ([1, 1], 2)])
+def test_nn2b_species_dimer_closed_form(types, code):
+    n, d = 5, 2.5
+    w = _nn2b_species_params(n, 6)
+    case = make_case(4, [[10, 10, 10], [10 + d, 10, 10]], types, cutoff=8.5, n=n)
+    d = fdist(d)
+    w0, b0 = w[:n].astype(float), w[n:2 * n].astype(float)
+    W1 = w[2 * n:2 * n + n * n].astype(float).reshape(n, n)
+    b1, w2 = w[n * (n + 2):n * (n + 3)].astype(float), w[n * (n + 3):n * (n + 4)].astype(float)
+    ws = w[n * (n + 4) + 1:].astype(float)
+    x1 = np.tanh(w0 * d + ws * code - b0)
+    p = float(w2 @ np.tanh(W1 @ x1 - b1)) - float(w[n * (n + 4)])
+    assert energy(case, w) == pytest.approx(p, rel=1e-10, abs=1e-12)
+
+
+def test_nn2b_species_forces_are_energy_gradient():
+    n = 6
+    w = _nn2b_species_params(n, 7)
+    pos, H = periodic_diamond(4)
+    types = np.arange(len(pos)) % 2
+    case = make_case(4, None, None, cutoff=6.5, n=n,
+                     cells=[(pos, types, H), (pos * 1.01, types, H * 1.01)])
+    f, _, _ = case.find_force(w)
+    atoms = [0, 5, 17, 40]
+    fd = fd_forces(case, w, atoms)
+    N = case.N
+    got = np.array([[f[k * N + i] for k in range(3)] for i in atoms])
+    np.testing.assert_allclose(got, fd, atol=2e-4 * np.abs(f[:3 * N]).max() + 1e-6)
