@@ -113,7 +113,8 @@ def run_serve(args):
               flush=True)
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, len(jobs)))
     arrivals -= arrivals[0]
-    srv = gpuga_amd.Server(memory_budget_bytes=args.budget_gb * 1e9, max_active=16)
+    srv = gpuga_amd.Server(memory_budget_bytes=args.budget_gb * 1e9, max_active=16,
+                           serve_share=1 if args.share_gpu else 0)
     t0 = time.perf_counter()
     submitted = 0
     while True:
@@ -352,6 +353,8 @@ def main():
     ap.add_argument("--rate", type=float, default=2.0, help="serve: arrivals per second")
     ap.add_argument("--generations", type=int, default=20, help="serve: generations per job")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: device memory budget")
+    ap.add_argument("--share-gpu", action="store_true",
+                    help="serve: each running job's launches on its share of the GPU")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain", action="store_true", help="per-individual launches")
     ap.add_argument("--host-ga", action="store_true",

@@ -39,7 +39,8 @@ class Options(C.Structure):
     _fields_ = [("backend", C.c_int), ("chunk_cost", C.c_int), ("use_graph", C.c_int),
                 ("nn2b_bf16", C.c_int), ("nn2b_grid", C.c_int), ("verbose", C.c_int),
                 ("blocks_per_cu", C.c_int), ("pack_lanes", C.c_int), ("queues", C.c_int),
-                ("ga_batch", C.c_int), ("tail_split", C.c_int)]
+                ("ga_batch", C.c_int), ("tail_split", C.c_int),
+                ("serve_share", C.c_int)]
 
 
 class ConvertOptions(C.Structure):

@@ -65,6 +65,7 @@ FitnessOptions to_options(const gpuga_options* o)
   f.queues = o->queues;
   f.ga_batch = o->ga_batch;
   f.tail_split = o->tail_split;
+  f.serve_share = o->serve_share;
   return f;
 }
 
@@ -94,6 +95,7 @@ void gpuga_default_options(gpuga_options* opt)
   opt->queues = f.queues;
   opt->ga_batch = f.ga_batch;
   opt->tail_split = f.tail_split;
+  opt->serve_share = f.serve_share;
 }
 
 int gpuga_fitness_create(const char* input_dir, const gpuga_options* opt, gpuga_fitness** out)

@@ -150,8 +150,19 @@ void PopPlan::drop_ga_graphs()
     }
 }
 
+void PopPlan::drop_share_graphs()
+{
+  for (int j = 0; j < kShares; ++j) {
+    if (share_exec[j]) (void)hipGraphExecDestroy(share_exec[j]);
+    if (share_graph[j]) (void)hipGraphDestroy(share_graph[j]);
+    share_exec[j] = nullptr;
+    share_graph[j] = nullptr;
+  }
+}
+
 PopPlan::~PopPlan()
 {
+  drop_share_graphs();
   drop_ga_graphs();
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
@@ -840,26 +851,29 @@ PopPlan& Fitness::plan(int P)
   return ref;
 }
 
-void Fitness::launch_population(PopPlan& pl, hipStream_t st, const float* genes)
+void Fitness::launch_population(PopPlan& pl, hipStream_t st, const float* genes, int grid)
 {
   PopCtl ctl = pl.ctl;
   if (genes) ctl.genes = genes;
+  // (a persistent grid of any size drains the same queues; the results do
+  // not depend on it)
+  const int G = grid > 0 ? grid : pl.grid;
   switch (spec_.type) {
-  case POT_RI: launch_ri_population(ctl, ds_.view, ds_.ri_tab.get(), pl.grid, pl.block, st); break;
+  case POT_RI: launch_ri_population(ctl, ds_.view, ds_.ri_tab.get(), G, pl.block, st); break;
   case POT_NN2B:
   case POT_NN2B_SPECIES: {
     NN2BGrid g = nn2b_grid_desc();
     launch_nn2b_tabulate(ctl, g, pl.nn2b_table.get(), st);
-    launch_nn2b_population(ctl, ds_.view, g, pl.nn2b_table.get(), pl.grid, pl.block, st);
+    launch_nn2b_population(ctl, ds_.view, g, pl.nn2b_table.get(), G, pl.block, st);
     break;
   }
   default: {
     if (loop_path_) {
-      launch_tersoff_loop_population(ctl, ds_.view, loop_, pl.grid, pl.block, st);
+      launch_tersoff_loop_population(ctl, ds_.view, loop_, G, pl.block, st);
       break;
     }
     // the instantiation with the packed path only when the plan has packed items
-    launch_tersoff_population(ctl, ds_.view, nn_class(population_neighbors().max_nn), pl.grid,
+    launch_tersoff_population(ctl, ds_.view, nn_class(population_neighbors().max_nn), G,
                               pl.block,
                               pl.groups_last > 1, st);
     break;
@@ -877,7 +891,7 @@ void Fitness::check_genes(int P, const float* genes) const
       fail("genes must lie in [0, 1] (the outer cut-off radius gene exceeds 1)");
 }
 
-void Fitness::compute_async(int P, const float* genes)
+void Fitness::compute_async(int P, const float* genes, int share)
 {
   if (P < 1) fail("population size must be positive");
   check_genes(P, genes);
@@ -887,22 +901,12 @@ void Fitness::compute_async(int P, const float* genes)
   GPUGA_HIP(hipMemcpyAsync(pl.genes.get(), pl.h_genes.get(), sizeof(float) * (size_t)P * spec_.V,
                            hipMemcpyHostToDevice, st));
   if (timing_) GPUGA_HIP(hipEventRecord(ev0_, st));
-  launch_generation(pl, st);
+  launch_generation(pl, st, share);
   if (timing_) GPUGA_HIP(hipEventRecord(ev1_, st));
   GPUGA_HIP(hipMemcpyAsync(pl.h_out.get(), pl.out.get(),
                            sizeof(double) * 3 * P + sizeof(float) * P + sizeof(unsigned),
                            hipMemcpyDeviceToHost, st));
   pending_P_ = P;
-}
-
-void Fitness::launch_generation(PopPlan& pl, hipStream_t st)
-{
-  if (opt_.use_graph) {
-    capture(pl, st);
-    GPUGA_HIP(hipGraphLaunch(pl.exec, st));
-  } else {
-    launch_population(pl, st);
-  }
 }
 
 namespace {
@@ -926,6 +930,27 @@ void captured(hipStream_t st, F&& body, hipGraph_t* graph, hipGraphExec_t* exec)
 }
 
 } // namespace
+
+void Fitness::launch_generation(PopPlan& pl, hipStream_t st, int share)
+{
+  // share 1, 2, 4 or 8 (rounded up, at most 8): 1/share of the grid
+  int j = 0;
+  while (j < PopPlan::kShares && (1 << j) < share) ++j;
+  const int grid = std::max(1, (pl.grid + (1 << j) - 1) >> j);
+  if (!opt_.use_graph) {
+    launch_population(pl, st, nullptr, grid);
+    return;
+  }
+  if (j == 0) {
+    capture(pl, st);
+    GPUGA_HIP(hipGraphLaunch(pl.exec, st));
+    return;
+  }
+  if (!pl.share_exec[j - 1])
+    captured(st, [&] { launch_population(pl, st, nullptr, grid); }, &pl.share_graph[j - 1],
+             &pl.share_exec[j - 1]);
+  GPUGA_HIP(hipGraphLaunch(pl.share_exec[j - 1], st));
+}
 
 void Fitness::capture(PopPlan& pl, hipStream_t st)
 {
@@ -1061,6 +1086,7 @@ void Fitness::set_spin_limit(int spins)
     pl.ctl.spin_limit = spins;
     // the captured launches hold the old control block
     pl.drop_ga_graphs();
+    pl.drop_share_graphs();
     if (pl.exec) (void)hipGraphExecDestroy(pl.exec);
     if (pl.graph) (void)hipGraphDestroy(pl.graph);
     pl.exec = nullptr;

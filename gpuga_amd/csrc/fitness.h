@@ -28,6 +28,8 @@ struct FitnessOptions {
   int ga_batch = 0;       // generations per staged batch of the device GA; 0 = up to 32
   int tail_split = 0;     // configurations smaller than one work item are cut into
                           // pieces of 1/tail_split of an item (0, 1: not cut)
+  int serve_share = 0;    // serving: 1 = each job's launches sized to its share of
+                          // the GPU (by the jobs running); 0 = the whole GPU each
 };
 
 // Device-resident copy of one training set and its geometry tables.
@@ -78,6 +80,12 @@ struct PopPlan {
   PopCtl ctl{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // the launch on 1/2, 1/4, 1/8 of the grid (a served job sharing the GPU),
+  // captured when first used
+  static constexpr int kShares = 3;
+  hipGraph_t share_graph[kShares] = {};
+  hipGraphExec_t share_exec[kShares] = {};
+  void drop_share_graphs();
   // device GA of this population size; runs of generations (population
   // launch + GA step each) as graphs: ga_exec[k][j] is 2^j generations
   // starting on population buffer k (a generation on buffer b writes the
@@ -111,7 +119,9 @@ public:
   // Per-atom outputs of one physical parameter set (host arrays).
   void find_force(const float* physical, float* f, float* virial, float* pe);
   // Launch the population generation asynchronously / wait for it.
-  void compute_async(int P, const float* genes);
+  // share: the launch runs on 1/share of the plan's grid (rounded up to 1, 2,
+  // 4 or 8), for a GPU shared with other jobs; results do not depend on it
+  void compute_async(int P, const float* genes, int share = 1);
   void compute_wait(float* fitness, double* terms);
   // Tune the work-item size for population size P.
   int autotune(int P);
@@ -164,10 +174,11 @@ private:
   PopPlan& plan(int P);
   void build_plan(PopPlan& pl, int P, int chunk_cost);
   // the population launch on genes (default: the plan's buffer)
-  void launch_population(PopPlan& pl, hipStream_t st, const float* genes = nullptr);
+  void launch_population(PopPlan& pl, hipStream_t st, const float* genes = nullptr,
+                         int grid = 0);
   // n generations of the device GA from population buffer k, captured
   void capture_generations(PopPlan& pl, int k, int n, hipGraph_t* graph, hipGraphExec_t* exec);
-  void launch_generation(PopPlan& pl, hipStream_t st); // graph replay or direct
+  void launch_generation(PopPlan& pl, hipStream_t st, int share = 1); // graph or direct
   void capture(PopPlan& pl, hipStream_t st);           // the plan's graph, once
   double time_plan(int P, int chunk_cost, int reps);
   // after a launch whose status word is set: clear the plan's counters and

@@ -92,10 +92,10 @@ void Server::admit(Job& j)
   });
 }
 
-void Server::launch(Job& j)
+void Server::launch(Job& j, int share)
 {
   j.t_launch = std::chrono::steady_clock::now();
-  j.fit->compute_async(j.gp.population_size, j.ga->population().data());
+  j.fit->compute_async(j.gp.population_size, j.ga->population().data(), share);
   j.inflight = true;
 }
 
@@ -203,12 +203,19 @@ void Server::poll()
     queue_.pop_front();
     admit(j);
   }
-  // 4. start the next generation of every idle ready job
+  // 4. start the next generation of every idle ready job, with serve_share on
+  // its share of the GPU (the jobs running generations split the device);
+  // results do not depend on the share
+  // (at most 4: launches from more streams than the process's hardware
+  // queues, 4 by default, do not run at once)
+  int running = 0;
+  for (auto& jp : jobs_) running += jp->state == 1 && jp->ready;
+  const int share = opt_.serve_share ? std::max(1, std::min(running, 4)) : 1;
   for (auto& jp : jobs_) {
     Job& j = *jp;
     if (j.state == 1 && j.ready && !j.inflight) {
       try {
-        launch(j);
+        launch(j, share);
       } catch (const std::exception& e) {
         retire(j, false, e.what());
       }

@@ -5,7 +5,11 @@
 // bytes fit a budget (and a slot is free), and each admitted job advances one
 // generation at a time on its own HIP stream, so generations of different
 // jobs overlap on the device.  A finished job releases its memory and the
-// next queued job is admitted (continuous batching).  Reading a job's files
+// next queued job is admitted (continuous batching).  Optionally
+// (serve_share) the jobs running generations split the device: each launch
+// runs on 1/n of its grid for n running jobs (up to 4).  By default every
+// launch takes the whole GPU: launches from up to four streams overlap at
+// their edges, and that measured faster than splitting.  Reading a job's files
 // and setting it up (training set on the device, work plan, graph) run on
 // worker threads, so admissions never hold up the generations of running
 // jobs.  The work-item size of each job is a deterministic function of its
@@ -68,7 +72,7 @@ private:
   };
   void admit(Job& j);
   bool task_done(Job& j) const;
-  void launch(Job& j);
+  void launch(Job& j, int share);
   void complete(Job& j);
   void retire(Job& j, bool ok, const std::string& err = "");
 

@@ -37,6 +37,9 @@ typedef struct gpuga_options {
                         random decisions; 0 = up to 32 */
   int tail_split;    /* configurations smaller than one work item are cut into
                         pieces of 1/tail_split of an item (0, 1: not cut) */
+  int serve_share;   /* server: 1 = each running job's launches on 1/n of the
+                        GPU for n running jobs (up to 4); 0 = the whole GPU
+                        (default: measured faster) */
 } gpuga_options;
 
 /* Thread-local text of the last failure (src/error.cu:21-25 prints it). */

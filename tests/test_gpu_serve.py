@@ -29,16 +29,20 @@ JOBS = [("si_diamond", 6, 64, 32, 1), ("gaas", 5, 40, 20, 2), ("nn2b", 6, 70, 36
         ("si_example", 4, 130, 64, 4)]
 
 
-def run_jobs(G, dirs, budget=64e9, max_active=8):
-    s = G.Server(memory_budget_bytes=budget, max_active=max_active, nn2b_bf16=0)
+def run_jobs(G, dirs, budget=64e9, max_active=8, share=0):
+    s = G.Server(memory_budget_bytes=budget, max_active=max_active, nn2b_bf16=0,
+                 serve_share=share)
     ids = [s.submit(d, write_outputs=True) for d in dirs]
     s.drain()
     return s, ids
 
 
-def test_results_same_alone_and_in_the_mix(golden, tmp_path):
+@pytest.mark.parametrize("share", [0, 1])
+def test_results_same_alone_and_in_the_mix(golden, tmp_path, share):
+    """share = 1: in the mix each running job's launches use 1/n of the GPU
+    (a different grid than alone); the results must not change."""
     import gpuga_amd as G
-    print_case("serve alone vs mix")
+    print_case("serve alone vs mix", share)
     alone, mixed = {}, {}
     for name, gens, P, par, seed in JOBS:
         d = job_dir(golden, str(tmp_path / "alone"), name, gens, P, par, seed)
@@ -47,7 +51,7 @@ def test_results_same_alone_and_in_the_mix(golden, tmp_path):
         alone[name] = (info, open(os.path.join(d, "ga.out")).read())
         assert info["state"] == 2 and info["generations"] == gens
     dirs = [job_dir(golden, str(tmp_path / "mix"), *j) for j in JOBS]
-    s, ids = run_jobs(G, dirs)
+    s, ids = run_jobs(G, dirs, share=share)
     for (name, gens, *_), jid, d in zip(JOBS, ids, dirs):
         info = s.job(jid, 2000)
         assert info["state"] == 2
