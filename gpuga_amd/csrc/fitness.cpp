@@ -783,7 +783,8 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   pl.bondf.resize(tersoff ? std::max<size_t>((size_t)K * (B_force + ts_.N_force) * 3 * WAVE, 1) : 1);
   if (spec_.nn2b())
     pl.nn2b_table.resize((size_t)spec_.pair_codes() * K * nn2b_grid_desc().G * 3 * WAVE);
-  const int groups = (Nc + CFG_GROUP - 1) / CFG_GROUP;
+  const int cfg_group = config_group(Nc);
+  const int groups = (Nc + cfg_group - 1) / cfg_group;
   pl.gpart.resize((size_t)K * groups * 3 * WAVE);
   // counters: cnt [Nc*K] | cnt2 [Nc*K] | grp_done [K*groups] | blk_done [K] |
   // (pad to 4 KiB) fetch [nq, kFetchStride apart] | done
@@ -826,6 +827,7 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   c.fitness = reinterpret_cast<float*>(c.terms + 3 * (size_t)P);
   c.status = reinterpret_cast<unsigned*>(c.fitness + P);
   c.gpart = pl.gpart.get();
+  c.cfg_group = cfg_group;
   c.spin_limit = spin_limit_;
   c.norm = norm();
   GPUGA_HIP(hipStreamSynchronize(st));

@@ -38,7 +38,8 @@
 //
 // Reductions.  The last chunk of a (configuration, lane block) folds the chunk
 // partials in chunk order and writes the configuration's error terms; the last
-// configuration of a group of 64 sums the group in index order; the last group
+// configuration of a group (about sqrt(Nc) of them) sums the group in index
+// order; the last group
 // of a lane block sums the groups and writes the fitness.  Every reduction has
 // a fixed order, so results do not depend on scheduling or on what else runs
 // on the device.  All counters are returned to zero by their last user, so the
@@ -53,7 +54,15 @@
 
 namespace gpuga {
 
-constexpr int CFG_GROUP = 64; // configurations per reduction group
+// Configurations per reduction group: about sqrt(Nc), a power of two in
+// [8, 64], so that the group fold and the lane-block fold over the groups
+// are both a few memory round trips.
+inline int config_group(int Nc)
+{
+  int g = 8;
+  while (g < 64 && g * g < Nc) g <<= 1;
+  return g;
+}
 constexpr int kTraceFields = 6; // words per item of the optional timeline
 constexpr int kQueuesPerXcd = 4; // work queues per XCD
 constexpr int kXcds = 8;         // XCDs of an MI355X
@@ -89,6 +98,7 @@ struct PopCtl {
   int B_force;
   double* err;            // [K][3][Nc][64]
   double* gpart;          // [K][groups][3][64]
+  int cfg_group;          // configurations per reduction group
   unsigned* cnt;          // [Nc][K]  phase-0 chunks done
   unsigned* cnt2;         // [Nc][K]  phase-1 chunks done
   unsigned* grp_done;     // [K][groups]
@@ -210,9 +220,10 @@ __device__ inline void config_done(const PopCtl& ctl, int Nc, int Nc_force, int 
     cstore(e, eE);
     cstore(e + E3, eW);
   }
-  const int groups = (Nc + CFG_GROUP - 1) / CFG_GROUP;
-  const int g = c / CFG_GROUP;
-  const int g0 = g * CFG_GROUP, gn = min(CFG_GROUP, Nc - g0);
+  const int cg = ctl.cfg_group;
+  const int groups = (Nc + cg - 1) / cg;
+  const int g = c / cg;
+  const int g0 = g * cg, gn = min(cg, Nc - g0);
   publish();
   const unsigned gd = wave_atomic_add(&ctl.grp_done[k * groups + g], 1u);
   if (gd != (unsigned)gn - 1u) return;
