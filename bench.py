@@ -292,6 +292,11 @@ def trace_split(fit, genes, n_force_cfg):
                                          float((ce[i] - begin) / 1e3)],
                           "end_us": float((t1[i] - begin) / 1e3)} for i in last]
     out["compute_end_us_max"] = float((np.nanmax(ce) - begin) / 1e3)
+    # when the waves that took an item entered the kernel (dispatch spread)
+    ent = (entry[~np.isnan(entry)] - begin) / 1e3
+    if ent.size:
+        out["wave_entry_us"] = {q: float(np.percentile(ent, v)) for q, v in
+                                (("p10", 10), ("p50", 50), ("p90", 90), ("max", 100))}
     # speed by dispatch round: workgroup b of the grid is resident on its CU
     # with b mod (grid / occupancy) as the slot; round = b // (workgroups per
     # round) orders the waves of one SIMD by age

@@ -591,22 +591,45 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   };
   std::vector<Chunk> chunks;
   std::vector<int> nch(Nc), slot0(Nc);
-  int slots = 0;
-  for (int c = 0; c < Nc; ++c) {
-    const int a0 = ts_.Na_sum[c], a1 = a0 + ts_.Na[c];
-    const bool fc = c < ts_.Nc_force;
-    long long total = 0;
-    for (int i = a0; i < a1; ++i) total += atom_cost(spec_.type, pn.NN[i], fc);
-    int n = (int)std::max<long long>(1, (total + chunk_cost - 1) / std::max(chunk_cost, 1));
+  std::vector<long long> cfg_total(Nc, 0);
+  for (int c = 0; c < Nc; ++c)
+    for (int i = ts_.Na_sum[c]; i < ts_.Na_sum[c] + ts_.Na[c]; ++i)
+      cfg_total[c] += atom_cost(spec_.type, pn.NN[i], c < ts_.Nc_force);
+  // chunks of configuration c for an item cost of `cost`
+  auto chunks_of = [&](int c, long long cost) {
+    const long long total = cfg_total[c];
+    cost = std::max<long long>(cost, 1);
+    int n = (int)std::max<long long>(1, (total + cost - 1) / cost);
     // A configuration smaller than one item comes at the end of its queue
     // (largest first), where the launch drains: it is cut into pieces of
     // 1/tail_split of an item, so that the last items are short.  (Not the
     // three-body force configurations: they come first.)
-    if (total < chunk_cost && !(tersoff && fc)) {
-      const long long piece = std::max<long long>(1, chunk_cost / std::max(tail_split, 1));
+    if (total < cost && !(tersoff && c < ts_.Nc_force)) {
+      const long long piece = std::max<long long>(1, cost / std::max(tail_split, 1));
       n = (int)std::max<long long>(1, (total + piece - 1) / piece);
     }
-    n = std::min(n, a1 - a0);
+    return std::min(n, ts_.Na[c]);
+  };
+  // A three-body force configuration runs two dependent steps (its chunks,
+  // then their force assembly), an energy configuration one.  When every item
+  // has a SIMD of its own (a launch too small to fill the device, so its time
+  // is the longest chain rather than the total work), the force
+  // configurations' chunks are cut at half the item cost, so that both
+  // chains end together.
+  long long force_cost = chunk_cost;
+  if (tersoff && ts_.Nc_force > 0) {
+    long long n_items = 0;
+    for (int c = 0; c < Nc; ++c)
+      n_items += (long long)chunks_of(c, c < ts_.Nc_force ? chunk_cost / 2 : chunk_cost) *
+                 (c < ts_.Nc_force ? 2 : 1);
+    if (n_items * K <= 4LL * num_cu_) force_cost = std::max(chunk_cost / 2, 1);
+  }
+  int slots = 0;
+  for (int c = 0; c < Nc; ++c) {
+    const int a0 = ts_.Na_sum[c], a1 = a0 + ts_.Na[c];
+    const bool fc = c < ts_.Nc_force;
+    const long long total = cfg_total[c];
+    const int n = chunks_of(c, fc ? force_cost : chunk_cost);
     nch[c] = n;
     slot0[c] = slots;
     slots += n;
@@ -768,6 +791,24 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   hipStream_t st = stream_;
   pl.items.upload(items, st);
   pl.qstart.upload(qstart, st);
+  // When the grid has a wave for every item, the first wave_slots are spread
+  // over the queues in proportion to their lengths (wave w starts at the
+  // queue that holds item w), so that each wave's first fetch finds an item.
+  // Queues are balanced by cost, not count: with home queues by XCD alone, a
+  // queue longer than its share of waves left its last items to waves that
+  // had gone through the empty queues first (about 2.6 us per queue tried),
+  // and the phase-0 items of small force configurations started up to 7 us
+  // late.
+  // (kept until the stream synchronisation below: the copy reads it)
+  std::vector<int> home;
+  int n_home = 0;
+  if ((long long)pl.n_items <= (long long)pl.grid * waves_per_block) {
+    home.resize(pl.n_items);
+    for (int q = 0; q < nq; ++q)
+      for (int i = qstart[q]; i < qstart[q + 1]; ++i) home[i] = q;
+    pl.home.upload(home, st);
+    n_home = pl.n_items;
+  }
   pl.cfg_nchunks.upload(nch, st);
   pl.cfg_slot0.upload(slot0, st);
   pl.part.resize((size_t)std::max(slots, 1) * K * 8 * WAVE);
@@ -803,6 +844,8 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   c.n_items = pl.n_items;
   c.nq = nq;
   c.qstart = pl.qstart.get();
+  c.home = n_home > 0 ? pl.home.get() : nullptr;
+  c.n_home = n_home;
   c.K = K;
   c.P = P;
   c.V = spec_.V;

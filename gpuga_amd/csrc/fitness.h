@@ -70,6 +70,7 @@ struct PopPlan {
   int nq = 1;          // work queues
   DeviceBuffer<WorkItem> items;
   DeviceBuffer<int> qstart;
+  DeviceBuffer<int> home; // first queue per wave slot (launches with a wave per item)
   DeviceBuffer<int> cfg_nchunks, cfg_slot0;
   DeviceBuffer<double> part, err, gpart;
   DeviceBuffer<double> out; // terms [P][3] | fitness [P] (float) | status (unsigned)

@@ -86,6 +86,8 @@ struct PopCtl {
   int n_items;
   int nq;                 // queues
   const int* qstart;      // [nq + 1] first item of each queue
+  const int* home;        // [n_home] first queue of wave slot w (else by XCD)
+  int n_home;
   int K;                  // lane blocks
   int P, V;
   const int* cfg_nchunks; // [Nc]
@@ -423,6 +425,13 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
   // spread over its queues)
   int q = (int)((xcc_id() * kQueuesPerXcd + (blockIdx.x / kXcds) % kQueuesPerXcd) %
                 (unsigned)ctl.nq);
+  // a launch with a wave for every item: each queue has as many home waves
+  // as items, so every first fetch finds one
+  // (wave_slot is the same in every lane, but derived from threadIdx.x the
+  // compiler cannot tell: taken through readfirstlane, the load is scalar
+  // and q stays in an SGPR, as do the item and everything that follows)
+  const int slot_u = wave_first((int)wave_slot);
+  if (slot_u < ctl.n_home) q = ldu(ctl.home, slot_u);
   for (;;) {
     const int it = next_item(ctl, q);
     unsigned long long* tr = ctl.trace ? ctl.trace + (long)kTraceFields * it : nullptr;
