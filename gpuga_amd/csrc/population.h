@@ -208,10 +208,10 @@ __device__ inline void sum_rows(const double* const (&a)[NA], long stride, int n
 // Configuration c of lane block k has its error terms (force configurations
 // the force term eF only, the others energy and virial eE, eW): propagate
 // through the group and lane-block reductions, writing the fitness when the
-// lane block is complete.  Only the terms a configuration has are stored
-// and read back.
+// lane block is complete (then it returns true).  Only the terms a
+// configuration has are stored and read back.
 template <int VALS>
-__device__ inline void config_done(const PopCtl& ctl, int Nc, int Nc_force, int c, int k,
+__device__ inline bool config_done(const PopCtl& ctl, int Nc, int Nc_force, int c, int k,
                                    int lane, double eE, double eW, double eF)
 {
   const long E3 = (long)Nc * WAVE;
@@ -228,7 +228,7 @@ __device__ inline void config_done(const PopCtl& ctl, int Nc, int Nc_force, int 
   const int g0 = g * cg, gn = min(cg, Nc - g0);
   publish();
   const unsigned gd = wave_atomic_add(&ctl.grp_done[k * groups + g], 1u);
-  if (gd != (unsigned)gn - 1u) return;
+  if (gd != (unsigned)gn - 1u) return false;
   observe();
   if (lane == 0) ctl.grp_done[k * groups + g] = 0u;
   // the group's force configurations [g0, fe), then its others [fe, g0 + gn)
@@ -246,7 +246,7 @@ __device__ inline void config_done(const PopCtl& ctl, int Nc, int Nc_force, int 
   cstore(gp + 2 * WAVE, s[2]);
   publish();
   const unsigned bd = wave_atomic_add(&ctl.blk_done[k], 1u);
-  if (bd != (unsigned)groups - 1u) return;
+  if (bd != (unsigned)groups - 1u) return false;
   observe();
   if (lane == 0) ctl.blk_done[k] = 0u;
   const double* gb = ctl.gpart + (long)k * groups * 3 * WAVE + lane;
@@ -263,6 +263,7 @@ __device__ inline void config_done(const PopCtl& ctl, int Nc, int Nc_force, int 
     ctl.terms[3 * (long)p + 1] = tW;
     ctl.terms[3 * (long)p + 2] = tF;
   }
+  return true;
 }
 
 // Store a chunk partial and, for the last chunk to arrive, fold all chunk
@@ -301,16 +302,17 @@ __device__ inline bool fold_chunks(const PopCtl& ctl, unsigned* counter, int c, 
       for (int j = 0; j < B; ++j) acc.v[7] += a[j];
     }
   } else {
-    // two chunks' partials (14 values) per round trip, added in chunk order
-    for (int ch = 0; ch < nch; ch += 2) {
-      double a[7], b[7];
+    // three chunks' partials (21 values) per round trip, added in chunk order
+    for (int ch = 0; ch < nch; ch += 3) {
+      double a[7], b[7], d[7];
 #pragma unroll
       for (int u = 0; u < 7; ++u) {
         a[u] = cload(base + ch * stride + u * WAVE);
         b[u] = ch + 1 < nch ? cload(base + (ch + 1) * stride + u * WAVE) : 0.0;
+        d[u] = ch + 2 < nch ? cload(base + (ch + 2) * stride + u * WAVE) : 0.0;
       }
 #pragma unroll
-      for (int u = 0; u < 7; ++u) acc.v[u] = (acc.v[u] + a[u]) + b[u];
+      for (int u = 0; u < 7; ++u) acc.v[u] = ((acc.v[u] + a[u]) + b[u]) + d[u];
     }
   }
   return true;
@@ -432,8 +434,12 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
   // and q stays in an SGPR, as do the item and everything that follows)
   const int slot_u = wave_first((int)wave_slot);
   if (slot_u < ctl.n_home) q = ldu(ctl.home, slot_u);
+  bool skip_fetch = false;
   for (;;) {
-    const int it = next_item(ctl, q);
+    // (readfirstlane: the flag is the same in every lane, which the compiler
+    // cannot tell, and a per-lane item index would move all the item's
+    // values out of scalar registers)
+    const int it = wave_first(skip_fetch ? -1 : next_item(ctl, q));
     unsigned long long* tr = ctl.trace ? ctl.trace + (long)kTraceFields * it : nullptr;
     if (ctl.trace && lane == 0) {
       const unsigned long long now = wall_clock64();
@@ -557,7 +563,13 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
         eW += dW * dW;
       }
     }
-    config_done<Pot::kFoldValues>(ctl, Nc, set.Nc_force, c, k, lane, eE, eW, eF);
+    // With one lane block, its fitness is written only once every item of
+    // the launch has finished, so every queue is empty: this wave goes
+    // straight to the exit (two memory round trips less at the end of the
+    // launch's longest chain).
+    if (config_done<Pot::kFoldValues>(ctl, Nc, set.Nc_force, c, k, lane, eE, eW, eF) &&
+        ctl.K == 1)
+      skip_fetch = true;
   }
 }
 
