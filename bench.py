@@ -392,8 +392,11 @@ def main():
                             tail_split=args.tail_split)
     P, V, N = w["P"], fit.V, fit.N
     chunk = args.chunk_cost
-    if not args.plain and chunk == 0:
-        chunk = fit.autotune(P)
+    shape = None
+    if not args.plain:
+        # (tunes the shape first when --chunk-cost is 0)
+        shape = fit.launch_shape(P)
+        chunk = shape["chunk_cost"]
     parents = P // 2 + (P - P // 2) % 2  # the GA pairs the children: P - parents even
     ga = gpuga_amd.GA(P, parents, V, maximum_generation=args.warmup + args.steps,
                       mutation_rate=0.2, seed=12345678)
@@ -481,7 +484,8 @@ def main():
                    "atoms": N, "bonds": fit.bonds, "parameters": V,
                    "backend": "per-individual" if args.plain else "population",
                    "ga": "device" if device_ga else "host",
-                   "chunk_cost": chunk, "s_per_generation": ms_step / 1e3,
+                   "chunk_cost": chunk, "launch_shape": shape,
+                   "s_per_generation": ms_step / 1e3,
                    "best_fitness": best},
         "roofline": {"bound": bound, "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak,

@@ -82,6 +82,7 @@ def lib():
             "gpuga_fitness_find_force": (C.c_int, [vp, fp, fp, fp, fp]),
             "gpuga_fitness_neighbors": (C.c_int, [vp, ip, ip, ip]),
             "gpuga_fitness_autotune": (C.c_int, [vp, C.c_int, ip]),
+            "gpuga_fitness_launch_shape": (C.c_int, [vp, C.c_int, ip, ip, ip, ip]),
             "gpuga_inspect_neighbors": (C.c_int, [C.c_char_p, ip, ip, ip, ip]),
             "gpuga_fitness_set_timing": (C.c_int, [vp, C.c_int]),
             "gpuga_fitness_timing": (C.c_int, [vp, dp, P(C.c_long), C.c_int]),
@@ -237,6 +238,13 @@ class Fitness:
         c = C.c_int()
         _check(lib().gpuga_fitness_autotune(self._h, int(P), C.byref(c)))
         return c.value
+
+    def launch_shape(self, P):
+        """The population launch's shape for P: chunk_cost, queues,
+        blocks_per_cu, packed (tuned first when the options ask for it)."""
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().gpuga_fitness_launch_shape(self._h, int(P), *[C.byref(x) for x in v]))
+        return dict(zip(("chunk_cost", "queues", "blocks_per_cu", "packed"), (x.value for x in v)))
 
     def run_ga(self, ga, first_generation, generations):
         """GA generations with population and GA on the device; returns the

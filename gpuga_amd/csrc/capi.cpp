@@ -371,6 +371,21 @@ int gpuga_fitness_autotune(gpuga_fitness* h, int P, int* chunk_cost)
   });
 }
 
+int gpuga_fitness_launch_shape(gpuga_fitness* h, int P, int* chunk_cost, int* queues,
+                               int* blocks_per_cu, int* packed)
+{
+  return guarded([&] {
+    need(h, "handle");
+    if (P < 1) throw std::invalid_argument("population size must be positive");
+    int c = 0, q = 0, b = 0, g = 0;
+    h->f->plan_shape(P, c, q, b, g);
+    if (chunk_cost) *chunk_cost = c;
+    if (queues) *queues = q;
+    if (blocks_per_cu) *blocks_per_cu = b;
+    if (packed) *packed = g > 1 ? 1 : 0;
+  });
+}
+
 int gpuga_fitness_set_timing(gpuga_fitness* h, int on)
 {
   return guarded([&] {

@@ -571,8 +571,19 @@ size_t Fitness::plan_bytes(int P) const
   return b;
 }
 
-void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
+LaunchShape Fitness::option_shape() const
 {
+  LaunchShape sh;
+  sh.chunk_cost = opt_.chunk_cost;
+  sh.queues = opt_.queues;
+  sh.blocks_per_cu = opt_.blocks_per_cu;
+  sh.pack = opt_.pack_lanes;
+  return sh;
+}
+
+void Fitness::build_plan(PopPlan& pl, int P, const LaunchShape& sh)
+{
+  const int chunk_cost = std::max(sh.chunk_cost, 1);
   const int Nc = ts_.Nc;
   const bool tersoff = spec_.type == POT_TERSOFF1 || spec_.type == POT_TERSOFF2;
   // neighbour counts of the list the kernel reads (Tersoff: the pruned one)
@@ -673,7 +684,7 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   // G = 2, P = 216, the launch took 0.607 ms packed against 0.521 ms
   // unpacked; with G = 4, P = 208, 0.458 against 0.523 ms.)
   int G_last = 1;
-  if (tersoff && opt_.pack_lanes && !loop_path_) {
+  if (tersoff && sh.pack && !loop_path_) {
     const int Pr = P - (K - 1) * WAVE;
     int W = 1;
     while (W < Pr) W <<= 1;
@@ -711,7 +722,7 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
   pl.n_items = (int)(eval.size() + assemble.size());
   const int waves_per_block = pl.block / WAVE;
   {
-    int occ = opt_.blocks_per_cu;
+    int occ = sh.blocks_per_cu;
     if (occ <= 0) {
       switch (spec_.type) {
       case POT_RI: occ = ri_population_occupancy(pl.block); break;
@@ -728,16 +739,18 @@ void Fitness::build_plan(PopPlan& pl, int P, int chunk_cost)
     }
     const int need = (pl.n_items + waves_per_block - 1) / waves_per_block;
     pl.grid = std::max(1, std::min(need, num_cu_ * occ));
+    pl.occ = occ;
   }
 
-  // Queues, kQueuesPerXcd per XCD (queue q is served first by XCD
-  // q / kQueuesPerXcd): whole configurations, balanced by cost (largest
-  // configuration to the least loaded queue).  Within a queue, phase-0 items
+  // Queues, nq / 8 per XCD (queue q is served first by XCD q / (nq / 8);
+  // four per XCD unless the shape says otherwise): whole configurations,
+  // balanced by cost (largest configuration to the least loaded queue).
+  // Within a queue, phase-0 items
   // largest chunk first, the lane blocks of a chunk together; a force-assembly
   // item follows the last phase-0 item of its configuration by about one
   // item per resident wave of the queue.
   // (at most 64: the kernel keeps one bit per queue in its closed mask)
-  const int nq = std::max(1, std::min({opt_.queues > 0 ? opt_.queues : kQueues,
+  const int nq = std::max(1, std::min({sh.queues > 0 ? sh.queues : kQueues,
                                        pl.grid * waves_per_block, 64}));
   std::vector<long long> cfg_cost(Nc, 0);
   for (const Costed& e : eval) cfg_cost[e.w.c] += e.cost;
@@ -884,13 +897,8 @@ PopPlan& Fitness::plan(int P)
 {
   auto it = plans_.find(P);
   if (it != plans_.end()) return *it->second;
-  int cost = opt_.chunk_cost;
-  if (cost <= 0) {
-    auto t = tuned_.find(P);
-    cost = t != tuned_.end() ? t->second : autotune(P);
-  }
   auto pl = std::make_unique<PopPlan>();
-  build_plan(*pl, P, cost);
+  build_plan(*pl, P, launch_shape(P));
   PopPlan& ref = *pl;
   plans_[P] = std::move(pl);
   return ref;
@@ -1394,10 +1402,11 @@ std::vector<double> Fitness::trace(int P, const float* genes)
   return rows;
 }
 
-double Fitness::time_plan(int P, int chunk_cost, int reps)
+double Fitness::time_plan(int P, const LaunchShape& sh, int reps, int* occ)
 {
   PopPlan pl;
-  build_plan(pl, P, chunk_cost);
+  build_plan(pl, P, sh);
+  if (occ) *occ = pl.occ;
   std::vector<float> genes((size_t)P * spec_.V);
   for (size_t i = 0; i < genes.size(); ++i) genes[i] = (float)((i * 2654435761u) % 1000) / 1000.0f;
   hipStream_t st = stream_;
@@ -1413,12 +1422,38 @@ double Fitness::time_plan(int P, int chunk_cost, int reps)
   return ms / reps;
 }
 
+LaunchShape Fitness::launch_shape(int P)
+{
+  if (opt_.chunk_cost > 0) return option_shape();
+  auto t = tuned_.find(P);
+  if (t == tuned_.end()) {
+    autotune(P);
+    t = tuned_.find(P);
+  }
+  return t->second;
+}
+
+void Fitness::plan_shape(int P, int& chunk_cost, int& queues, int& blocks_per_cu,
+                         int& groups_last)
+{
+  if (P < 1) fail("population size must be positive");
+  const PopPlan& pl = plan(P);
+  chunk_cost = pl.chunk_cost;
+  queues = pl.nq;
+  blocks_per_cu = pl.occ;
+  groups_last = pl.groups_last;
+}
+
 int Fitness::autotune(int P)
 {
   // Candidate work-item sizes: the whole-configuration cost spread over
   // 1/2 .. 1/32 of the resident waves, and the largest configuration's cost
   // halved step by step (small sets, where latency rather than balance
-  // sets the time).
+  // sets the time).  Then, at the best size, the rest of the launch shape
+  // that the options leave open: the queue count, one resident workgroup
+  // per CU fewer than the occupancy calculator allows, and, when the last
+  // lane block qualifies, packing it or not.  Each choice is kept only if
+  // its launch is faster.
   const bool tersoff = spec_.type == POT_TERSOFF1 || spec_.type == POT_TERSOFF2;
   const std::vector<int>& pnn = tersoff ? population_neighbors().NN : nl_.NN;
   long long total = 0, maxcfg = 0;
@@ -1442,28 +1477,68 @@ int Fitness::autotune(int P)
   for (long long c = maxcfg / 2; c >= 64; c /= 2) cands.push_back((int)c);
   std::sort(cands.begin(), cands.end());
   cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
-  int best = cands.back();
+  LaunchShape best = option_shape();
+  best.chunk_cost = cands.back();
   double best_ms = 1e30;
-  auto consider = [&](int c, int reps) {
-    const double ms = time_plan(P, c, reps);
-    if (opt_.verbose) std::printf("autotune P=%d chunk_cost=%d: %.4f ms\n", P, c, ms);
+  int best_occ = 0;
+  auto consider = [&](const LaunchShape& sh, int reps) {
+    int occ = 0;
+    const double ms = time_plan(P, sh, reps, &occ);
+    if (opt_.verbose)
+      std::printf("autotune P=%d chunk_cost=%d queues=%d blocks_per_cu=%d pack=%d: %.4f ms\n", P,
+                  sh.chunk_cost, sh.queues, sh.blocks_per_cu, sh.pack, ms);
     if (ms < best_ms) {
       best_ms = ms;
-      best = c;
+      best = sh;
+      best_occ = occ;
     }
   };
-  for (int c : cands) consider(c, 3);
-  // then the neighbourhood of the best (the coarse candidates are a factor
-  // of two apart, and launch times differ by about 1% from run to run), the
+  auto with_cost = [&](int c) {
+    LaunchShape sh = best;
+    sh.chunk_cost = c;
+    return sh;
+  };
+  const LaunchShape base = best;
+  for (int c : cands) {
+    LaunchShape sh = base;
+    sh.chunk_cost = c;
+    consider(sh, 3);
+  }
+  // the neighbourhood of the best (the coarse candidates are a factor of
+  // two apart, and launch times differ by about 1% from run to run), the
   // best one timed again alongside
-  const int coarse = best;
+  const int coarse = best.chunk_cost;
   best_ms = 1e30;
   for (double f : {0.6, 0.75, 0.87, 1.0, 1.15, 1.33, 1.6}) {
     const long long c = std::llround(coarse * f);
-    if (c >= 64 && c <= std::max<long long>(maxcfg, 64)) consider((int)c, 6);
+    if (c >= 64 && c <= std::max<long long>(maxcfg, 64)) consider(with_cost((int)c), 6);
+  }
+  // queue count (the kernel keeps at most 64)
+  if (opt_.queues <= 0) {
+    const LaunchShape inc = best;
+    for (int nq : {kQueues / 2, kQueues * 2}) {
+      LaunchShape sh = inc;
+      sh.queues = std::min(nq, 64);
+      consider(sh, 6);
+    }
+  }
+  // one resident workgroup per CU fewer (fewer waves sharing a SIMD)
+  if (opt_.blocks_per_cu <= 0 && best_occ > 1) {
+    LaunchShape sh = best;
+    sh.blocks_per_cu = best_occ - 1;
+    consider(sh, 6);
+  }
+  // the packed last lane block against plain lanes
+  if (opt_.pack_lanes && tersoff && !loop_path_) {
+    const int Pr = P - (K - 1) * WAVE;
+    if (Pr <= 16) {
+      LaunchShape sh = best;
+      sh.pack = 0;
+      consider(sh, 6);
+    }
   }
   tuned_[P] = best;
-  return best;
+  return best.chunk_cost;
 }
 
 } // namespace gpuga

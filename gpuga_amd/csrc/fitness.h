@@ -32,6 +32,17 @@ struct FitnessOptions {
                           // the GPU (by the jobs running); 0 = the whole GPU each
 };
 
+// Shape of the population launch for one population size: work-item cost,
+// queue count, resident workgroups per CU and lane packing.  0 in queues or
+// blocks_per_cu means the built-in choice (four queues per XCD; the
+// occupancy calculator).
+struct LaunchShape {
+  int chunk_cost = 0;
+  int queues = 0;
+  int blocks_per_cu = 0;
+  int pack = 1; // pack a last lane block of <= 16 individuals
+};
+
 // Device-resident copy of one training set and its geometry tables.
 struct DeviceSet {
   DeviceBuffer<int> Na, Na_sum, type, nn, bond0, nl, rev, ptype, tri0;
@@ -66,6 +77,7 @@ std::unique_ptr<Potential> make_potential(const PotentialSpec& spec, const Devic
 // One population plan (work items, scratch) for a population size P.
 struct PopPlan {
   int P = 0, K = 0, n_items = 0, grid = 0, block = 256, chunk_cost = 0;
+  int occ = 0;         // resident workgroups per CU the grid is sized for
   int groups_last = 1; // lane groups of the last lane block's items
   int nq = 1;          // work queues
   DeviceBuffer<WorkItem> items;
@@ -124,8 +136,15 @@ public:
   // 4 or 8), for a GPU shared with other jobs; results do not depend on it
   void compute_async(int P, const float* genes, int share = 1);
   void compute_wait(float* fitness, double* terms);
-  // Tune the work-item size for population size P.
+  // Tune the launch shape for population size P (the parts the options
+  // leave open) and return its work-item cost.
   int autotune(int P);
+  // The launch shape the population size P runs with (tuning it first when
+  // the options ask for that).
+  LaunchShape launch_shape(int P);
+  // What the plan for P was built with: work-item cost, queues, resident
+  // workgroups per CU, lane groups of the last lane block.
+  void plan_shape(int P, int& chunk_cost, int& queues, int& blocks_per_cu, int& groups_last);
   // Build the work plan (and its graph) for population size P now, so the
   // first generation does not pay for it.
   void prepare(int P);
@@ -173,7 +192,9 @@ public:
 private:
   void setup();
   PopPlan& plan(int P);
-  void build_plan(PopPlan& pl, int P, int chunk_cost);
+  void build_plan(PopPlan& pl, int P, const LaunchShape& sh);
+  // the shape the options fix (chunk cost > 0: no tuning)
+  LaunchShape option_shape() const;
   // the population launch on genes (default: the plan's buffer)
   void launch_population(PopPlan& pl, hipStream_t st, const float* genes = nullptr,
                          int grid = 0);
@@ -181,7 +202,7 @@ private:
   void capture_generations(PopPlan& pl, int k, int n, hipGraph_t* graph, hipGraphExec_t* exec);
   void launch_generation(PopPlan& pl, hipStream_t st, int share = 1); // graph or direct
   void capture(PopPlan& pl, hipStream_t st);           // the plan's graph, once
-  double time_plan(int P, int chunk_cost, int reps);
+  double time_plan(int P, const LaunchShape& sh, int reps, int* occ = nullptr);
   // after a launch whose status word is set: clear the plan's counters and
   // results, then fail
   [[noreturn]] void discard_launch(PopPlan& pl);
@@ -212,7 +233,7 @@ private:
   DeviceSet ds_;
   std::unique_ptr<Potential> potential_;
   std::map<int, std::unique_ptr<PopPlan>> plans_;
-  std::map<int, int> tuned_;
+  std::map<int, LaunchShape> tuned_;
   hipStream_t stream_ = nullptr;
   int device_ = 0;
   int num_cu_ = 0;

@@ -64,7 +64,7 @@ inline int config_group(int Nc)
   return g;
 }
 constexpr int kTraceFields = 6; // words per item of the optional timeline
-constexpr int kQueuesPerXcd = 4; // work queues per XCD
+constexpr int kQueuesPerXcd = 4; // work queues per XCD (default)
 constexpr int kXcds = 8;         // XCDs of an MI355X
 constexpr int kQueues = kQueuesPerXcd * kXcds;
 // distance of two queue counters in 32-bit words (4 KiB + 256 B: distinct
@@ -425,8 +425,8 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
   const unsigned long long t_entry = ctl.trace ? wall_clock64() : 0ull;
   // home queue: one of the queues of this wave's XCD (workgroups of one XCD
   // spread over its queues)
-  int q = (int)((xcc_id() * kQueuesPerXcd + (blockIdx.x / kXcds) % kQueuesPerXcd) %
-                (unsigned)ctl.nq);
+  const unsigned qpx = (unsigned)max(ctl.nq / kXcds, 1); // queues per XCD
+  int q = (int)((xcc_id() * qpx + (blockIdx.x / kXcds) % qpx) % (unsigned)ctl.nq);
   // a launch with a wave for every item: each queue has as many home waves
   // as items, so every first fetch finds one
   // (wave_slot is the same in every lane, but derived from threadIdx.x the

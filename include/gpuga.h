@@ -150,8 +150,17 @@ int gpuga_expand_train_in(const char* in_path, const char* out_path, const int* 
 int gpuga_fitness_geometry(const gpuga_fitness* h, int* sizes3, float* bond, float* tri, float* rec);
 int gpuga_inspect_geometry(const char* input_dir, int* sizes3, float* bond, float* tri, float* rec);
 
-/* Work-item size chosen by timing candidate sizes for population P. */
+/* Launch shape chosen by timing candidates for population P (work-item
+ * size, then the queue count, resident workgroups per CU and lane packing
+ * that the options leave at 0 / on); returns the work-item size.  No
+ * counterpart in the reference (its launch shape is fixed). */
 int gpuga_fitness_autotune(gpuga_fitness* h, int P, int* chunk_cost);
+/* The shape the population launch for P runs with (tuned first when the
+ * options ask for it): work-item cost, queues, resident workgroups per CU
+ * and 1 if the last lane block is packed.  Any pointer may be NULL.  No
+ * counterpart in the reference. */
+int gpuga_fitness_launch_shape(gpuga_fitness* h, int P, int* chunk_cost, int* queues,
+                               int* blocks_per_cu, int* packed);
 
 /* HIP-event timing of the population launches (or of each individual's
  * launches on the per-individual backend) on the library's stream. */

@@ -170,6 +170,37 @@ def test_chunking_does_not_change_results(G, golden):
     np.testing.assert_allclose(ta, tb, rtol=1e-12)
 
 
+@pytest.mark.parametrize("name", ["si_diamond", "hfo2"])
+def test_launch_shape_does_not_change_results(G, golden, name):
+    """Queue count and resident workgroups per CU only change which wave
+    takes which item; every fold has a fixed order, so the results are the
+    same bits.  The autotuned shape, given back as options, reproduces the
+    tuned handle's results bit for bit."""
+    print_case("launch shape", name)
+    P = 130
+    with G.Fitness(golden(name), chunk_cost=200) as base:
+        genes = genes_for(base.V, P, 91)
+        ref, tref = base.compute(genes, terms=True)
+    for opts in ({"queues": 16}, {"queues": 64}, {"queues": 3}, {"blocks_per_cu": 1},
+                 {"blocks_per_cu": 2, "queues": 64}):
+        with G.Fitness(golden(name), chunk_cost=200, **opts) as f:
+            got, tgot = f.compute(genes, terms=True)
+            assert f.launch_shape(P)["chunk_cost"] == 200
+        assert np.array_equal(tgot, tref), opts
+        assert np.array_equal(got, ref), opts
+    with G.Fitness(golden(name)) as tuned:
+        sh = tuned.launch_shape(P)
+        assert sh["chunk_cost"] >= 64 and 1 <= sh["queues"] <= 64 and sh["blocks_per_cu"] >= 1
+        a, ta = tuned.compute(genes, terms=True)
+    with G.Fitness(golden(name), chunk_cost=sh["chunk_cost"], queues=sh["queues"],
+                   blocks_per_cu=sh["blocks_per_cu"], pack_lanes=sh["packed"]) as fixed:
+        b, tb = fixed.compute(genes, terms=True)
+    assert np.array_equal(ta, tb)
+    # (another item size regroups the FP64 sums, and packing the last lane
+    # block sums its lane groups by a butterfly: the packed-lane bound)
+    np.testing.assert_allclose(ta, tref, rtol=1e-5)
+
+
 @pytest.mark.parametrize("name", ["si_diamond", "gaas", "hfo2"])
 def test_cross_wave_handoff_stress(G, golden, name):
     """Tiny work items split every configuration into many chunks, so chunk
