@@ -23,7 +23,7 @@ struct FitnessOptions {
   int nn2b_grid = 1024;   // NN2B tabulation points
   int verbose = 0;
   int blocks_per_cu = 0;  // 0 = from the occupancy calculator
-  int pack_lanes = 1;     // pack a last lane block of <= 32 individuals
+  int pack_lanes = 1;     // pack a last lane block of <= 16 individuals
   int queues = 0;         // work queues of the population launch; 0 = one per XCD
   int ga_batch = 0;       // generations per staged batch of the device GA; 0 = up to 32
   int tail_split = 0;     // configurations smaller than one work item are cut into

@@ -23,16 +23,21 @@ typedef struct gpuga_server gpuga_server;
 typedef struct gpuga_options {
   int backend;    /* 0: one population launch per generation (default);
                      1: per-individual launches, as src/fitness.cu:318-332 */
-  int chunk_cost; /* work-item size of the population launch; 0 = autotune */
+  int chunk_cost; /* work-item size of the population launch; 0 = autotune
+                     the launch shape (this and the fields below left at 0
+                     or, for pack_lanes, at 1) */
   int use_graph;  /* 1: replay the population launch from a HIP graph */
   int nn2b_bf16;  /* 1: NN2B hidden layer on BF16 MFMA; 0: FP32 */
   int nn2b_grid;  /* NN2B tabulation points (default 1024) */
   int verbose;
   int blocks_per_cu; /* resident workgroups per CU of the population launch;
-                        0 = as many as the occupancy calculator allows */
-  int pack_lanes;    /* 1: a last lane block of <= 32 individuals runs several
-                        atoms per wave (three-body potentials); 0: one atom */
-  int queues;        /* work queues of the population launch; 0 = one per XCD */
+                        0 = as many as the occupancy calculator allows (or
+                        one fewer, when the autotuner finds that faster) */
+  int pack_lanes;    /* 1: a last lane block of <= 16 individuals runs several
+                        atoms per wave (three-body potentials; the autotuner
+                        may decide against it); 0: one atom */
+  int queues;        /* work queues of the population launch; 0 = four per
+                        XCD (or 16 / 64, as the autotuner finds faster) */
   int ga_batch;      /* generations per staged batch of gpuga_fitness_run_ga's
                         random decisions; 0 = up to 32 */
   int tail_split;    /* configurations smaller than one work item are cut into
