@@ -121,7 +121,9 @@ def test_species_pair_input_with_zero_weights_is_the_reference_form():
     d3.update(potential_type=3, pmin=d4["pmin"][: n * (n + 4) + 1],
               pmax=d4["pmax"][: n * (n + 4) + 1])
     out = []
+    # (one fixed item size: two autotuned handles may pick different sizes,
+    # which regroups the FP64 sums)
     for d, g in ((d3, genes4[:, : n * (n + 4) + 1]), (d4, genes4)):
-        with G.Fitness(arrays=d, nn2b_bf16=0) as fit:
+        with G.Fitness(arrays=d, nn2b_bf16=0, chunk_cost=200) as fit:
             out.append(fit.compute(g, terms=True))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
