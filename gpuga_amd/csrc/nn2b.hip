@@ -155,6 +155,7 @@ struct NN2BPop {
     }
   }
 
+  static constexpr bool kHeaderAhead = false;
   __device__ static void chunk(const DevSet& s, const Lane&, const WorkItem& it, const PopCtl&,
                                const Extra& ex, Acc& acc)
   {

@@ -204,6 +204,7 @@ struct RIPop {
   // Energy/virial configurations visit each unordered pair once (the table
   // holds only j > i for their atoms: the reverse bond carries the same d and
   // -r); force configurations visit every bond so F_i is complete in the lane.
+  static constexpr bool kHeaderAhead = false;
   __device__ static void chunk(const DevSet& s, const RILane& q, const WorkItem& it,
                                const PopCtl&, const float4*, Acc& acc)
   {

@@ -399,6 +399,10 @@ __device__ inline double group_sum(double x, int W)
 //   struct Lane; typedef Extra;                     per-lane constants, tables
 //   static Lane make(ctl, p);                       denormalised genes of p
 //   static void chunk(set, q, item, ctl, ex, Acc&)  evaluate atoms [a0, a1)
+//   static constexpr bool kHeaderAhead;             chunk<true> exists: a form
+//                                                   that loads ahead (used by
+//                                                   the instantiation without
+//                                                   packed items)
 //   static constexpr bool kForcePass;               three-body force assembly
 //   static double force_pass(set, ctl, ex, a0, a1, k, lane)
 //                                                   sum |F-F_ref|^2 over [a0, a1)
@@ -497,7 +501,10 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
         const int p = min(k * WAVE + lane, ctl.P - 1);
         const typename Pot::Lane q = Pot::make(ctl, p);
         mark(4);
-        Pot::chunk(set, q, item, ctl, ex, acc);
+        if constexpr (Pot::kHeaderAhead && !PACK)
+          Pot::template chunk<true>(set, q, item, ctl, ex, acc);
+        else
+          Pot::chunk(set, q, item, ctl, ex, acc);
         mark(5);
       }
       if (Pot::kForcePass && is_force) {

@@ -249,16 +249,55 @@ struct TersoffPop {
   static constexpr int REV = 1 + NMAX + NMAX * NMAX; // reverse-bond indices
   static constexpr int INV = REV + NMAX / 4;          // inverse distances
 
+  // record header: neighbour count, bond-type bits, longest bond
+  struct Hdr {
+    int nn, unl;
+    float dmax;
+  };
+
   template <bool FORCE, bool U>
   __device__ static void atom(const DevSet& s, const TLane& q, int i, float* fblk, int lane,
                               Acc& acc)
   {
     if constexpr (U) i = __builtin_amdgcn_readfirstlane(i);
-    const float4* R = s.rec + (long)i * RS;
     // header words as int loads (an int4 load through the constant address
     // space is issued as a vector load, waited for at every atom)
-    const int* H = reinterpret_cast<const int*>(R);
-    const int nn = ldx<U>(H, 0), unl = TWO ? ldx<U>(H, 1) : 0;
+    const int* H = reinterpret_cast<const int*>(s.rec + (long)i * RS);
+    atom_h<FORCE, U>(s, q, i, Hdr{ldx<U>(H, 0), TWO ? ldx<U>(H, 1) : 0, __int_as_float(ldx<U>(H, 3))},
+                     fblk, lane, acc);
+  }
+
+  // The next atom's record, touched by one vector load while the current
+  // atom runs (a scalar load would be waited for at once: the kernel is
+  // short of SGPRs, so the compiler moves such values to VGPR lanes as soon
+  // as they arrive).  Lanes 0-2 read the header words (count, longest bond,
+  // bond types), lanes 3-5 one word in each later 128-byte line of the
+  // record, which brings those lines into L2 for the row loads.
+  __device__ static int ahead_load(const DevSet& s, int i)
+  {
+    const char* base = reinterpret_cast<const char*>(s.rec + (long)i * RS);
+    const int lane = __lane_id();
+    const int off = lane == 1   ? 12
+                    : lane == 2 ? 4
+                    : lane == 3 ? min(128, RS * 16 - 4)
+                    : lane == 4 ? min(256, RS * 16 - 4)
+                    : lane == 5 ? RS * 16 - 4
+                                : 0;
+    return *reinterpret_cast<const int*>(base + off);
+  }
+  __device__ static Hdr header_of(int v)
+  {
+    return Hdr{__builtin_amdgcn_readlane(v, 0), TWO ? __builtin_amdgcn_readlane(v, 2) : 0,
+               __int_as_float(__builtin_amdgcn_readlane(v, 1))};
+  }
+
+  // the atom with its header already loaded
+  template <bool FORCE, bool U>
+  __device__ static void atom_h(const DevSet& s, const TLane& q, int i, const Hdr& h,
+                                float* fblk, int lane, Acc& acc)
+  {
+    const float4* R = s.rec + (long)i * RS;
+    const int nn = h.nn, unl = h.unl;
     // an atom with the full neighbour bound (every atom of diamond Si) runs
     // a body without neighbour-count guards: no selects, no guarded loads;
     // when its longest bond is also shorter than every lane's R1 (the usual
@@ -267,7 +306,7 @@ struct TersoffPop {
     // for a third copy)
     if constexpr (U) {
       if (nn == NMAX) {
-        if (NMAX <= 4 && __int_as_float(ldx<U>(H, 3)) < q.r1min)
+        if (NMAX <= 4 && h.dmax < q.r1min)
           atom_body<FORCE, U, true, NMAX <= 4>(s, q, i, R, nn, unl, fblk, lane, acc);
         else
           atom_body<FORCE, U, true, false>(s, q, i, R, nn, unl, fblk, lane, acc);
@@ -458,21 +497,48 @@ struct TersoffPop {
     }
   }
 
+  // HDR: load the next atom's header during the current atom (four-bond
+  // records; the kernel instantiation without the packed path, whose
+  // register budget has room for it)
+  static constexpr bool kHeaderAhead = true;
+  template <bool HDR = false>
   __device__ static void chunk(const DevSet& s, const TLane& q, const WorkItem& it,
                                const PopCtl& ctl, int, Acc& acc)
   {
     const int lane = __lane_id();
     const unsigned r = dispatch_round();
-    if (it.c < s.Nc_force) {
-      float* fblk = ctl.bondf + (size_t)it.k * ctl.B_force * 3 * WAVE;
-      for (int i = it.a0; i < it.a1; ++i) {
-        rotate_priority(r + (unsigned)i);
-        atom<true, true>(s, q, i, fblk, lane, acc);
+    if constexpr (NMAX > 4 || !HDR) {
+      if (it.c < s.Nc_force) {
+        float* fblk = ctl.bondf + (size_t)it.k * ctl.B_force * 3 * WAVE;
+        for (int i = it.a0; i < it.a1; ++i) {
+          rotate_priority(r + (unsigned)i);
+          atom<true, true>(s, q, i, fblk, lane, acc);
+        }
+      } else {
+        for (int i = it.a0; i < it.a1; ++i) {
+          rotate_priority(r + (unsigned)i);
+          atom<false, true>(s, q, i, nullptr, lane, acc);
+        }
       }
     } else {
-      for (int i = it.a0; i < it.a1; ++i) {
-        rotate_priority(r + (unsigned)i);
-        atom<false, true>(s, q, i, nullptr, lane, acc);
+      // the next atom's header is loaded during this atom, so its row loads
+      // issue at once (one memory latency per atom, not two)
+      int hv = ahead_load(s, it.a0);
+      if (it.c < s.Nc_force) {
+        float* fblk = ctl.bondf + (size_t)it.k * ctl.B_force * 3 * WAVE;
+        for (int i = it.a0; i < it.a1; ++i) {
+          rotate_priority(r + (unsigned)i);
+          const Hdr h = header_of(hv);
+          hv = ahead_load(s, min(i + 1, it.a1 - 1));
+          atom_h<true, true>(s, q, i, h, fblk, lane, acc);
+        }
+      } else {
+        for (int i = it.a0; i < it.a1; ++i) {
+          rotate_priority(r + (unsigned)i);
+          const Hdr h = header_of(hv);
+          hv = ahead_load(s, min(i + 1, it.a1 - 1));
+          atom_h<false, true>(s, q, i, h, nullptr, lane, acc);
+        }
       }
     }
   }
@@ -676,6 +742,7 @@ struct TersoffLoop {
     }
   }
 
+  static constexpr bool kHeaderAhead = false;
   __device__ static void chunk(const DevSet& s, const TLane& q, const WorkItem& it,
                                const PopCtl& ctl, const LoopTab& t, Acc& acc)
   {
