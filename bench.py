@@ -375,6 +375,8 @@ def main():
                          "pieces per item (0: the library default)")
     ap.add_argument("--trace-split", action="store_true",
                     help="add the work-item timeline split of one population launch")
+    ap.add_argument("--population", type=int, default=0,
+                    help="population size (0: the workload's own)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the population-sharded multi-GPU form "
@@ -390,7 +392,7 @@ def main():
     d, w = make_workload(args.workload)
     fit = gpuga_amd.Fitness(arrays=d, backend=1 if args.plain else 0, chunk_cost=args.chunk_cost,
                             tail_split=args.tail_split)
-    P, V, N = w["P"], fit.V, fit.N
+    P, V, N = (args.population or w["P"]), fit.V, fit.N
     chunk = args.chunk_cost
     shape = None
     if not args.plain:

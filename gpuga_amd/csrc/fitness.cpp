@@ -980,6 +980,10 @@ void captured(hipStream_t st, F&& body, hipGraph_t* graph, hipGraphExec_t* exec)
   }
   GPUGA_HIP(hipStreamEndCapture(st, graph));
   GPUGA_HIP(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+  // the graph's upload to the device, now: left to its first launch, it
+  // held that launch back by up to 2 ms with the device idle (an 8-
+  // generation graph of the 384-individual Tersoff workload)
+  GPUGA_HIP(hipGraphUpload(*exec, st));
 }
 
 } // namespace
@@ -1060,6 +1064,14 @@ void Fitness::run_ga(GA& ga, int first_generation, int generations, float* best,
   // device starts after one generation's draws and the host then draws each
   // batch while the device runs the one before
   const int B = dga.batch();
+  // Every graph a batch can use (runs of 2^j generations on either
+  // population buffer) is captured before the first launch, on the first
+  // call: captured on first use, they cost the call that met them about
+  // 1-3 ms of host time with the device idle.
+  if (opt_.use_graph)
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < PopPlan::kGaGraphs && (1 << j) <= B; ++j)
+        if (!pl.ga_exec[k][j]) capture_generations(pl, k, 1 << j, &pl.ga_graph[k][j], &pl.ga_exec[k][j]);
   const int H = dga.history_cap();
   std::vector<float> h((size_t)H * (1 + V));
   int g = 0; // generations queued
