@@ -470,13 +470,17 @@ void DeviceGA::stage(GA& ga, int generation, int n, hipStream_t st)
     r[0] = m;
     w += 1 + 3 * (size_t)pairs_ + (size_t)groups_ + 1 + 2 * (size_t)m;
   }
-  // In pieces of 8 KiB: one host-to-device copy of a few tens of KiB, queued
-  // behind a running batch, held the host for milliseconds (6 ms measured
-  // for 27 KB with four generations queued; 8 KiB pieces return at once),
-  // and the device then waited for the next batch's launch.
-  constexpr size_t kPiece = 2048; // words
-  for (size_t o = 0; o < w; o += kPiece)
-    GPUGA_HIP(hipMemcpyAsync(draws_.get() + o, h + o, sizeof(int) * std::min(kPiece, w - o),
+  // In pieces of at least 16 KiB, at most 16 of them: one host-to-device
+  // copy of a few tens of KiB queued behind a running batch held the host
+  // for milliseconds (6 ms measured for 27 KB with four generations queued,
+  // 32 KiB pieces as well; 8 and 16 KiB pieces return at once), and the
+  // device then waited for the next batch's launch.  Batches of megabytes
+  // (the NN2B workload's 59 000 mutations per generation) are drawn more
+  // slowly than the device runs them, and each further piece costs a few
+  // microseconds of host time, so their pieces are larger.
+  const size_t piece = std::max<size_t>(4096, (w + 15) / 16); // words
+  for (size_t o = 0; o < w; o += piece)
+    GPUGA_HIP(hipMemcpyAsync(draws_.get() + o, h + o, sizeof(int) * std::min(piece, w - o),
                              hipMemcpyHostToDevice, st));
   GPUGA_HIP(hipEventRecord(copied_[k], st));
   pending_[k] = true;
