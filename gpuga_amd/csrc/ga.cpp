@@ -319,15 +319,15 @@ DeviceGA::DeviceGA(const GA& ga, float* genes0, int history_cap, hipStream_t st,
   hist_gen0_.resize(1);
   counters_.resize(2); // steps | workgroups done
   counters_.zero(st);
-  // a record holds at most the mutations of generation 0 (the count falls
-  // with the generation) and at most one per gene outside the best row
+  // a record: the mutation count, the crossover words, the mutations' gene
+  // indices and values; at most the mutations of generation 0 (the count
+  // falls with the generation)
   pairs_ = (P - p.parent_number) / 2;
-  max_mut_ = std::min(ga.mutation_count(0), (int)(PV - (size_t)V));
+  max_mut_ = ga.mutation_count(0);
   int rows = 0;
   size_t lds = 0;
   ga_step_shape(P, V, &rows, &lds);
-  groups_ = (P + rows - 1) / rows; // workgroups of a step
-  const size_t rec_max = 1 + 3 * (size_t)pairs_ + (size_t)groups_ + 1 + 2 * (size_t)max_mut_;
+  const size_t rec_max = 1 + 3 * (size_t)pairs_ + 2 * (size_t)max_mut_;
   const size_t budget = (size_t)1 << 22; // words per batch
   const size_t most = max_batch > 0 ? std::min(max_batch, kGaMaxBatch) : kGaMaxBatch;
   const int batch = (int)std::max<size_t>(1, std::min<size_t>(most, budget / rec_max));
@@ -338,7 +338,6 @@ DeviceGA::DeviceGA(const GA& ga, float* genes0, int history_cap, hipStream_t st,
     host_[k].resize(stage_words_);
     GPUGA_HIP(hipEventCreateWithFlags(&copied_[k], hipEventDisableTiming));
   }
-  seen_.assign(PV, 0);
   d_.P = P;
   d_.V = V;
   d_.parents = p.parent_number;
@@ -413,72 +412,24 @@ void DeviceGA::stage(GA& ga, int generation, int n, hipStream_t st)
     val_.clear();
     ga.draw_crossover(cross_);
     const int count = ga.draw_mutation(generation + i, idx_, val_);
+    if (count > max_mut_) fail("device GA: more mutations than the staging area holds");
+    // in draw order; the step keeps the last one of each gene, as the
+    // reference's sequential writes do
     int* r = h + w;
+    r[0] = count;
     std::memcpy(r + 1, cross_.data(), sizeof(int) * 3 * (size_t)pairs_);
-    // Grouped by the workgroup whose rows they fall in (each workgroup reads
-    // its own range), and within a group the last mutation of a gene is the
-    // one that stays.  Three cache-friendly passes: group counts, a stable
-    // scatter by group, then per group a backward walk that keeps each gene's
-    // first sighting (a stamp per gene of the group's rows).
-    int* bo = r + 1 + 3 * pairs_; // [groups + 1]
-    const long span = (long)d_.rows * d_.V; // genes per workgroup
-    const double inv_span = 1.0 / (double)span;
-    fill_.assign(groups_ + 1, 0);
-    kgrp_.resize(count);
-    for (int j = 0; j < count; ++j) {
-      const int g = idx_[j];
-      // g / span by a multiplication, corrected to the exact quotient
-      int q = groups_ == 1 ? 0 : (int)((double)g * inv_span);
-      if ((long)q * span > g) --q;
-      else if ((long)(q + 1) * span <= g) ++q;
-      kgrp_[j] = q;
-      ++fill_[q + 1];
-    }
-    for (int q = 0; q < groups_; ++q) fill_[q + 1] += fill_[q];
-    kidx_.resize(count);
-    keep_.resize(count);
-    for (int j = 0; j < count; ++j) {
-      const int at = fill_[kgrp_[j]]++;
-      kidx_[at] = idx_[j];
-      keep_[at] = val_[j];
-    }
-    // fill_[q] is now the end of group q's entries; the kept indices go
-    // straight into the record, their values into val_ (free by now), since
-    // the values follow all the indices
-    int* ri = bo + groups_ + 1;
-    int m = 0;
-    for (int q = 0; q < groups_; ++q) {
-      bo[q] = m;
-      const int begin = q == 0 ? 0 : fill_[q - 1], end = fill_[q];
-      if (++seen_tag_ == INT_MAX) {
-        std::fill(seen_.begin(), seen_.end(), 0);
-        seen_tag_ = 1;
-      }
-      const long base = (long)q * span;
-      for (int j = end - 1; j >= begin; --j) {
-        const int local = (int)(kidx_[j] - base);
-        if (seen_[local] == seen_tag_) continue;
-        seen_[local] = seen_tag_;
-        if (m >= max_mut_) fail("device GA: more mutations than the staging area holds");
-        ri[m] = kidx_[j];
-        val_[m] = keep_[j];
-        ++m;
-      }
-    }
-    bo[groups_] = m;
-    std::memcpy(ri + m, val_.data(), sizeof(float) * (size_t)m);
-    r[0] = m;
-    w += 1 + 3 * (size_t)pairs_ + (size_t)groups_ + 1 + 2 * (size_t)m;
+    std::memcpy(r + 1 + 3 * pairs_, idx_.data(), sizeof(int) * (size_t)count);
+    std::memcpy(r + 1 + 3 * pairs_ + count, val_.data(), sizeof(float) * (size_t)count);
+    w += 1 + 3 * (size_t)pairs_ + 2 * (size_t)count;
   }
-  // In pieces of at least 16 KiB, at most 16 of them: one host-to-device
-  // copy of a few tens of KiB queued behind a running batch held the host
-  // for milliseconds (6 ms measured for 27 KB with four generations queued,
-  // 32 KiB pieces as well; 8 and 16 KiB pieces return at once), and the
-  // device then waited for the next batch's launch.  Batches of megabytes
-  // (the NN2B workload's 59 000 mutations per generation) are drawn more
-  // slowly than the device runs them, and each further piece costs a few
-  // microseconds of host time, so their pieces are larger.
-  const size_t piece = std::max<size_t>(4096, (w + 15) / 16); // words
+  // In pieces of 16 KiB: one host-to-device copy of a few tens of KiB
+  // queued behind a running batch held the host until the device was done
+  // with that batch, or longer (6 ms measured for 27 KB with four
+  // generations queued, 32 KiB pieces as well; 8 and 16 KiB pieces return
+  // at once), and the device then waited for the next batch's launch.  (A
+  // piece costs about 5 us of host time: 0.15 ms for a generation of the
+  // NN2B workload's 59 000 mutations.)
+  constexpr size_t piece = 4096; // words
   for (size_t o = 0; o < w; o += piece)
     GPUGA_HIP(hipMemcpyAsync(draws_.get() + o, h + o, sizeof(int) * std::min(piece, w - o),
                              hipMemcpyHostToDevice, st));

@@ -18,13 +18,12 @@ class GA;
 // at most `batch` records: word 0 is the step (counted from the upload) that
 // reads the first record, words [1, batch] the record offsets (in words,
 // from the buffer start), and a record is
-//   [0]                      n = mutations after removing overwritten ones
+//   [0]                      n = mutations of the generation
 //   [1, 1 + 3 * pairs)       crossover: parent rank p1, p2, crossover point
-//   [.., + groups + 1)       where each workgroup's mutations start (the
-//                            mutations are grouped by the workgroup whose
-//                            rows they fall in), then n
-//   [.., + n)                gene index of each mutation
+//   [.., + n)                gene index of each mutation, in draw order
 //   [.., + n)                its value (float bits)
+// (A gene drawn more than once keeps its last value; the step resolves
+// that on the device.)
 // Step s reads record s - word 0 of the buffer staged before it.
 struct GaDev {
   int P, V, parents, children;
@@ -93,7 +92,7 @@ public:
 
 private:
   GaDev d_{};
-  int pairs_ = 0, max_mut_ = 0, max_gen_ = 0, groups_ = 1;
+  int pairs_ = 0, max_mut_ = 0, max_gen_ = 0;
   float rate_ = 0.0f;
   size_t stage_words_ = 0;
   long staged_ = 0; // records staged since the upload
@@ -106,9 +105,8 @@ private:
   hipEvent_t copied_[2] = {nullptr, nullptr};
   bool pending_[2] = {false, false};
   int next_ = 0;
-  std::vector<int> cross_, idx_, seen_, kidx_, kgrp_, fill_;
-  std::vector<float> val_, keep_;
-  int seen_tag_ = 0;
+  std::vector<int> cross_, idx_;
+  std::vector<float> val_;
 };
 
 } // namespace gpuga

@@ -173,14 +173,14 @@ __global__ void __launch_bounds__(GA_THREADS) ga_step(GaDev d)
   const int r1 = min(P, r0 + d.rows);
   const int nel = (r1 - r0) * V;
   const long e0 = (long)r0 * V;
+  // the generation's mutations in draw order: gene indices, then values
   const int nm = rec[0];
-  const int* bo = rec + 1 + 3 * pairs; // this workgroup's mutations: [bo[g], bo[g + 1])
-  const int* mi = bo + gridDim.x + 1;
+  const int* mi = rec + 1 + 3 * pairs;
   const float* mv = reinterpret_cast<const float*>(mi + nm);
-  const int j0 = bo[blockIdx.x], j1 = bo[blockIdx.x + 1];
-  // this thread's first mutation, loaded while the rows are gathered
-  const int mi0 = j0 + t < j1 ? mi[j0 + t] : -1;
-  const float mv0 = j0 + t < j1 ? mv[j0 + t] : 0.0f;
+  // per gene of this workgroup's rows, the last mutation that hits it (the
+  // reference applies them in draw order, so the last one stays)
+  int* win = reinterpret_cast<int*>(tile + (size_t)d.rows * V);
+  for (int e = t; e < nel; e += T) win[e] = -1;
   for (int e = t; e < nel; e += T) {
     const int n = r0 + e / V, m = e % V;
     int src;
@@ -195,14 +195,21 @@ __global__ void __launch_bounds__(GA_THREADS) ga_step(GaDev d)
     tile[e] = d.genes_in[(long)src * V + m];
   }
   __syncthreads();
-  if (mi0 >= 0) {
-    const long i = (long)mi0 - e0;
-    if (i >= 0 && i < nel) tile[i] = mv0;
+  // every workgroup scans all the mutations, four indices per thread and
+  // round trip
+  for (int j = 4 * t; j < nm; j += 4 * T) {
+    int g[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) g[u] = j + u < nm ? mi[j + u] : -1;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long i = (long)g[u] - e0;
+      if (g[u] >= 0 && i >= 0 && i < nel) atomicMax(&win[i], j + u);
+    }
   }
-  for (int j = j0 + t + T; j < j1; j += T) {
-    const long i = (long)mi[j] - e0;
-    if (i >= 0 && i < nel) tile[i] = mv[j];
-  }
+  __syncthreads();
+  for (int e = t; e < nel; e += T)
+    if (win[e] >= 0) tile[e] = mv[win[e]];
   __syncthreads();
   for (int e = t; e < nel; e += T) d.genes_out[e0 + e] = tile[e];
 
@@ -229,8 +236,10 @@ void ga_step_shape(int P, int V, int* rows, size_t* lds)
   const int r = V >= TILE_FLOATS ? 1 : TILE_FLOATS / V;
   *rows = r;
   const size_t pairs = (size_t)(P / 2); // at most: children < P
+  // sort keys and order, crossover words, the rows' tile and, per gene of
+  // the tile, the last mutation that hits it
   *lds = (sizeof(float) * 2 + sizeof(int)) * (size_t)Ppad + sizeof(int) * 3 * pairs +
-         sizeof(float) * (size_t)r * V;
+         (sizeof(float) + sizeof(int)) * (size_t)r * V;
 }
 
 void ga_step_prepare(int P, int V)
