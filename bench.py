@@ -87,30 +87,11 @@ def run_serve(args):
 
     import gpuga_amd
     from gpuga_amd import synth
-    rng = np.random.default_rng(2024)
-    kinds = ["tersoff", "ri", "nn2b"]
-    jobs = []
     root = tempfile.mkdtemp(prefix="gpuga_serve_")
-    for j in range(args.jobs):
-        kind = kinds[j % 3]
-        Nc = int(math.exp(rng.uniform(math.log(100), math.log(2000))))
-        P = int(rng.choice([64, 128, 192, 256]))
-        if kind == "tersoff":
-            d = synth.tersoff_si(Nc=Nc, sizes="loguniform", seed=100 + j)
-        elif kind == "ri":
-            d = synth.fluorite_ab2(Nc=max(2, Nc // 4), seed=100 + j)
-        else:
-            d = synth.wurtzite_ab(Nc=max(2, Nc // 2), seed=100 + j)
-        lab = gpuga_amd.Fitness(arrays=d)
-        d = synth.label(d, lab.find_force)
-        lab.close()
-        path = os.path.join(root, f"job{j:02d}_{kind}")
-        synth.write_dir(d, path, ga={"maximum_generation": args.generations,
-                                     "population_size": P, "parent_number": P // 2,
-                                     "mutation_rate": 0.2, "seed": 1000 + j})
-        jobs.append((path, kind, P, int(np.sum(d["Na"]))))
-        print(f"prepared job {j}: {kind} Nc={len(d['Na'])} N={jobs[-1][3]} P={P}", file=sys.stderr,
-              flush=True)
+    rng = np.random.default_rng(2024)
+    jobs = synth.served_mix(root, jobs=args.jobs, generations=args.generations, rng=rng,
+                            log=lambda m: print(m, file=sys.stderr, flush=True))
+    # arrival times continue the same stream
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, len(jobs)))
     arrivals -= arrivals[0]
     srv = gpuga_amd.Server(memory_budget_bytes=args.budget_gb * 1e9, max_active=16,

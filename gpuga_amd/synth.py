@@ -244,3 +244,39 @@ def write_dir(d, path, ga=None):
         with open(os.path.join(path, "ga.in"), "w") as f:
             for k, v in ga.items():
                 f.write(f"{k} {v}\n")
+
+
+def served_mix(root, jobs=16, generations=20, rng=None, log=None):
+    """The served mixed workload's jobs: minimal Tersoff Si, AB2 rigid ion
+    and NN2B wurtzite in turn, 100-2000 configurations (log-uniform; a
+    quarter of that for rigid ion, half for NN2B), populations of 64-256,
+    each labelled with its ground-truth parameters and written as a job
+    directory under root.  rng: the generator the sizes are drawn from
+    (default: seeded 2024).  Returns [(path, kind, P, atoms)]."""
+    import gpuga_amd
+    if rng is None:
+        rng = np.random.default_rng(2024)
+    kinds = ["tersoff", "ri", "nn2b"]
+    out = []
+    for j in range(jobs):
+        kind = kinds[j % 3]
+        Nc = int(math.exp(rng.uniform(math.log(100), math.log(2000))))
+        P = int(rng.choice([64, 128, 192, 256]))
+        if kind == "tersoff":
+            d = tersoff_si(Nc=Nc, sizes="loguniform", seed=100 + j)
+        elif kind == "ri":
+            d = fluorite_ab2(Nc=max(2, Nc // 4), seed=100 + j)
+        else:
+            d = wurtzite_ab(Nc=max(2, Nc // 2), seed=100 + j)
+        lab = gpuga_amd.Fitness(arrays=d)
+        d = label(d, lab.find_force)
+        lab.close()
+        path = os.path.join(root, f"job{j:02d}_{kind}")
+        write_dir(d, path, ga={"maximum_generation": generations, "population_size": P,
+                               "parent_number": P // 2, "mutation_rate": 0.2,
+                               "seed": 1000 + j})
+        out.append((path, kind, P, int(np.sum(d["Na"]))))
+        if log:
+            log(f"prepared job {j}: {kind} Nc={len(d['Na'])} N={out[-1][3]} P={P}")
+    return out
+

@@ -106,3 +106,32 @@ def test_failed_jobs_do_not_stop_the_others(golden, tmp_path):
     with pytest.raises(G.GpugaError, match="population_size should >= 20"):
         s.job(ids[2], 9)
     assert len(s.admissions()) == 1
+
+
+def test_served_mix_alone_equals_mix(tmp_path):
+    """The served mixed workload's own jobs (16: minimal Tersoff Si, AB2 rigid
+    ion and NN2B wurtzite in turn, 100-2000 configurations, P 64-256), two
+    generations each: every job's best fitness, elite and ga.out are the
+    same run alone and submitted together with all the others."""
+    import gpuga_amd as G
+    from gpuga_amd import synth
+    print_case("served mix, alone vs mix")
+    jobs = synth.served_mix(str(tmp_path / "alone"), jobs=16, generations=2)
+    assert {k for _, k, _, _ in jobs} == {"tersoff", "ri", "nn2b"}
+    alone = {}
+    for path, kind, P, N in jobs:
+        s, ids = run_jobs(G, [path])
+        info = s.job(ids[0], 2000)
+        assert info["state"] == 2 and info["generations"] == 2, (path, info)
+        alone[os.path.basename(path)] = (info, open(os.path.join(path, "ga.out")).read())
+        s.close()
+    mixed = synth.served_mix(str(tmp_path / "mix"), jobs=16, generations=2)
+    s, ids = run_jobs(G, [p for p, *_ in mixed], max_active=16)
+    for (path, *_), jid in zip(mixed, ids):
+        info = s.job(jid, 2000)
+        ref, ref_out = alone[os.path.basename(path)]
+        assert info["state"] == 2
+        assert info["best"] == ref["best"], path
+        assert np.array_equal(info["elite"], ref["elite"]), path
+        assert open(os.path.join(path, "ga.out")).read() == ref_out, path
+    s.close()
