@@ -629,10 +629,14 @@ void Fitness::build_plan(PopPlan& pl, int P, const LaunchShape& sh)
   // chains end together.
   long long force_cost = chunk_cost;
   if (tersoff && ts_.Nc_force > 0) {
+    // (a force configuration of several chunks has an assembly item per
+    // chunk; 4 SIMDs per CU)
     long long n_items = 0;
-    for (int c = 0; c < Nc; ++c)
-      n_items += (long long)chunks_of(c, c < ts_.Nc_force ? chunk_cost / 2 : chunk_cost) *
-                 (c < ts_.Nc_force ? 2 : 1);
+    for (int c = 0; c < Nc; ++c) {
+      const bool fc = c < ts_.Nc_force;
+      const int n = chunks_of(c, fc ? chunk_cost / 2 : chunk_cost);
+      n_items += (long long)n * (fc && n > 1 ? 2 : 1);
+    }
     if (n_items * K <= 4LL * num_cu_) force_cost = std::max(chunk_cost / 2, 1);
   }
   int slots = 0;
@@ -745,10 +749,9 @@ void Fitness::build_plan(PopPlan& pl, int P, const LaunchShape& sh)
   // Queues, nq / 8 per XCD (queue q is served first by XCD q / (nq / 8);
   // four per XCD unless the shape says otherwise): whole configurations,
   // balanced by cost (largest configuration to the least loaded queue).
-  // Within a queue, phase-0 items
-  // largest chunk first, the lane blocks of a chunk together; a force-assembly
-  // item follows the last phase-0 item of its configuration by about one
-  // item per resident wave of the queue.
+  // Within a queue, phase-0 items largest chunk first, the lane blocks of a
+  // chunk together; a force-assembly item follows the last phase-0 item of
+  // its configuration by about one item per resident wave of the queue.
   // (at most 64: the kernel keeps one bit per queue in its closed mask)
   const int nq = std::max(1, std::min({sh.queues > 0 ? sh.queues : kQueues,
                                        pl.grid * waves_per_block, 64}));

@@ -39,11 +39,11 @@
 // Reductions.  The last chunk of a (configuration, lane block) folds the chunk
 // partials in chunk order and writes the configuration's error terms; the last
 // configuration of a group (about sqrt(Nc) of them) sums the group in index
-// order; the last group
-// of a lane block sums the groups and writes the fitness.  Every reduction has
-// a fixed order, so results do not depend on scheduling or on what else runs
-// on the device.  All counters are returned to zero by their last user, so the
-// launch can be replayed from a captured graph without clearing memory.
+// order; the last group of a lane block sums the groups and writes the
+// fitness.  Every reduction has a fixed order, so results do not depend on
+// scheduling or on what else runs on the device.  All counters are returned
+// to zero by their last user, so the launch can be replayed from a captured
+// graph without clearing memory.
 //
 // Replaces the per-individual host loop of Fitness::compute
 // (src/fitness.cu:316-333) and its reductions (src/fitness.cu:392-555).
