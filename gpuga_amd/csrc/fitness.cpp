@@ -3,6 +3,7 @@
 #include "noalloc.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -382,26 +383,46 @@ void Fitness::setup()
     ds_.ri_tab.upload(tab, st);
     // population form: per atom the pairs it evaluates (all bonds for force
     // configurations, those with j > i for the others), grouped by pair type
-    // t_i + t_j, each as unit vector and d, then the factors plus 6 d^-7
+    // t_i + t_j.  A force configuration's pair: unit vector and d, then the
+    // factors plus 6 d^-7 (two float4); an energy configuration's pair: d and
+    // the virial coefficients u_a u_b d (xx, yy, zz, then xy, yz, zx), then
+    // the factors (three float4: the virial then takes one FMA per
+    // component).  The header's first word is the atom's first float4.
     const std::vector<int> p0 = pair_offsets();
-    std::vector<float4> pt(2 * (size_t)std::max(p0[ts_.N], 1));
+    auto rows_before = [&](int i) { // float4 rows of the atoms before i
+      const long nf = std::min(i, ts_.N_force);
+      return 2 * (long)p0[nf] + 3 * ((long)p0[i] - p0[nf]);
+    };
+    const long rows = rows_before(ts_.N);
+    if (rows > INT_MAX) fail("rigid-ion pair table too large");
+    std::vector<float4> pt((size_t)std::max<long>(rows, 1));
     std::vector<int4> hdr(std::max(ts_.N, 1));
     parallel_for(ts_.N, [&](int i) {
       const int nn = nl_.NN[i], b0 = nl_.start[i];
+      const bool force = i < ts_.N_force;
       int cnt[3] = {0, 0, 0};
-      float4* out = &pt[2 * (size_t)p0[i]];
+      const long first = rows_before(i);
+      float4* out = &pt[(size_t)first];
       for (int t = 0; t < 3; ++t)
         for (int u = 0; u < nn; ++u) {
           const int b = b0 + u;
           if (ptype[b] != t || !evaluates(i, b)) continue;
           const Vec4 r = geo_.bond[b];
-          const double dinv = 1.0 / (double)r.w;
+          const double d = r.w, dinv = 1.0 / d;
           const double d6 = std::pow(dinv, 6);
-          *out++ = make_float4((float)(r.x * dinv), (float)(r.y * dinv), (float)(r.z * dinv), r.w);
+          if (force) {
+            *out++ = make_float4((float)(r.x * dinv), (float)(r.y * dinv), (float)(r.z * dinv), r.w);
+          } else {
+            // u_a u_b d = r_a r_b / d
+            *out++ = make_float4(r.w, (float)(r.x * r.x * dinv), (float)(r.y * r.y * dinv),
+                                 (float)(r.z * r.z * dinv));
+            *out++ = make_float4((float)(r.x * r.y * dinv), (float)(r.y * r.z * dinv),
+                                 (float)(r.z * r.x * dinv), 0.0f);
+          }
           *out++ = make_float4((float)d6, (float)(6.0 * d6 * dinv), tab[b].y, tab[b].z);
           ++cnt[t];
         }
-      hdr[i] = make_int4(p0[i], cnt[0], cnt[1], cnt[2]);
+      hdr[i] = make_int4((int)first, cnt[0], cnt[1], cnt[2]);
     });
     ds_.pair_tab.upload(pt, st);
     ds_.pair_hdr.upload(hdr, st);

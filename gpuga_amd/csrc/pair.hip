@@ -155,49 +155,61 @@ struct RIPop {
   // E += U, W -= u (x) u d g over the unordered pairs (the others).  The
   // geometry is wave-uniform (scalar loads); four pairs per step (their
   // loads issue together and are waited for once), then two, then one.
+  // (float4 rows per pair: two for a force configuration's pair, three for
+  // an energy configuration's)
   template <int T, bool FORCE>
   __device__ static void pairs(const RILane& q, const float4* tab, int n, float* a)
   {
+    constexpr int R = FORCE ? 2 : 3;
     int k = 0;
     for (; k + 4 <= n; k += 4) {
-      const float4 u0 = ldu(tab, 2 * k), f0 = ldu(tab, 2 * k + 1);
-      const float4 u1 = ldu(tab, 2 * k + 2), f1 = ldu(tab, 2 * k + 3);
-      const float4 u2 = ldu(tab, 2 * k + 4), f2 = ldu(tab, 2 * k + 5);
-      const float4 u3 = ldu(tab, 2 * k + 6), f3 = ldu(tab, 2 * k + 7);
-      one<T, FORCE>(q, u0, f0, a);
-      one<T, FORCE>(q, u1, f1, a);
-      one<T, FORCE>(q, u2, f2, a);
-      one<T, FORCE>(q, u3, f3, a);
+      float4 r[4][R];
+#pragma unroll
+      for (int h = 0; h < 4; ++h)
+#pragma unroll
+        for (int m = 0; m < R; ++m) r[h][m] = ldu(tab, R * (k + h) + m);
+#pragma unroll
+      for (int h = 0; h < 4; ++h) one<T, FORCE>(q, r[h], a);
     }
     for (; k + 2 <= n; k += 2) {
-      const float4 u0 = ldu(tab, 2 * k), f0 = ldu(tab, 2 * k + 1);
-      const float4 u1 = ldu(tab, 2 * k + 2), f1 = ldu(tab, 2 * k + 3);
-      one<T, FORCE>(q, u0, f0, a);
-      one<T, FORCE>(q, u1, f1, a);
+      float4 r[2][R];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int m = 0; m < R; ++m) r[h][m] = ldu(tab, R * (k + h) + m);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) one<T, FORCE>(q, r[h], a);
     }
-    if (k < n) one<T, FORCE>(q, ldu(tab, 2 * k), ldu(tab, 2 * k + 1), a);
+    if (k < n) {
+      float4 r[R];
+#pragma unroll
+      for (int m = 0; m < R; ++m) r[m] = ldu(tab, R * k + m);
+      one<T, FORCE>(q, r, a);
+    }
   }
 
   template <int T, bool FORCE>
-  __device__ static __forceinline__ void one(const RILane& q, const float4& u, const float4& f,
+  __device__ static __forceinline__ void one(const RILane& q, const float4 (&r)[FORCE ? 2 : 3],
                                              float* a)
   {
     float p, g;
-    ri_pair_pop<T, !FORCE>(q, u.w, f, p, g);
-    if (FORCE) {
-      a[0] = fmaf(u.x, g, a[0]);
-      a[1] = fmaf(u.y, g, a[1]);
-      a[2] = fmaf(u.z, g, a[2]);
+    if constexpr (FORCE) {
+      // r[0] = (unit vector, d), r[1] = factors
+      ri_pair_pop<T, false>(q, r[0].w, r[1], p, g);
+      a[0] = fmaf(r[0].x, g, a[0]);
+      a[1] = fmaf(r[0].y, g, a[1]);
+      a[2] = fmaf(r[0].z, g, a[2]);
     } else {
-      const float t = u.w * g;
-      const float tx = u.x * t, ty = u.y * t, tz = u.z * t;
+      // r[0] = (d, u_x u_x d, u_y u_y d, u_z u_z d), r[1] = (u_x u_y d,
+      // u_y u_z d, u_z u_x d, 0), r[2] = factors
+      ri_pair_pop<T, true>(q, r[0].x, r[2], p, g);
       a[0] += p;
-      a[1] = fmaf(-u.x, tx, a[1]);
-      a[2] = fmaf(-u.y, ty, a[2]);
-      a[3] = fmaf(-u.z, tz, a[3]);
-      a[4] = fmaf(-u.x, ty, a[4]);
-      a[5] = fmaf(-u.y, tz, a[5]);
-      a[6] = fmaf(-u.z, tx, a[6]);
+      a[1] = fmaf(-r[0].y, g, a[1]);
+      a[2] = fmaf(-r[0].z, g, a[2]);
+      a[3] = fmaf(-r[0].w, g, a[3]);
+      a[4] = fmaf(-r[1].x, g, a[4]);
+      a[5] = fmaf(-r[1].y, g, a[5]);
+      a[6] = fmaf(-r[1].z, g, a[6]);
     }
   }
 
@@ -212,7 +224,7 @@ struct RIPop {
     if (it.c < s.Nc_force) {
       for (int i = it.a0; i < it.a1; ++i) {
         const int4 h = ldu(s.pair_hdr, i);
-        const float4* tab = s.pair_tab + 2 * (long)h.x;
+        const float4* tab = s.pair_tab + (long)h.x;
         float a[3] = {0.0f, 0.0f, 0.0f};
         pairs<0, true>(q, tab, h.y, a);
         pairs<1, true>(q, tab + 2 * (long)h.y, h.z, a);
@@ -224,11 +236,11 @@ struct RIPop {
     } else {
       for (int i = it.a0; i < it.a1; ++i) {
         const int4 h = ldu(s.pair_hdr, i);
-        const float4* tab = s.pair_tab + 2 * (long)h.x;
+        const float4* tab = s.pair_tab + (long)h.x;
         float a[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         pairs<0, false>(q, tab, h.y, a);
-        pairs<1, false>(q, tab + 2 * (long)h.y, h.z, a);
-        pairs<2, false>(q, tab + 2 * (long)(h.y + h.z), h.w, a);
+        pairs<1, false>(q, tab + 3 * (long)h.y, h.z, a);
+        pairs<2, false>(q, tab + 3 * (long)(h.y + h.z), h.w, a);
 #pragma unroll
         for (int u = 0; u < 7; ++u) acc.v[u] += (double)a[u];
       }
