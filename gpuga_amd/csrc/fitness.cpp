@@ -432,22 +432,34 @@ void Fitness::setup()
     // configurations, j > i for the others) in order of distance, so that
     // consecutive pairs read neighbouring rows of the tabulated network; per
     // pair (unit vector, d), the cubic Hermite weights on the tabulation
-    // grid (h00, h h10, h01, h h11), and (fc, fc', grid cell, 0) with the
-    // reference's fixed taper (src/nn2b.cu:80-92)
+    // grid (h00, h h10, h01, h h11), and (fc, fc', grid cell, species code)
+    // with the reference's fixed taper (src/nn2b.cu:80-92)
     const NN2BGrid grid = nn2b_grid_desc();
     const std::vector<int> p0 = pair_offsets();
-    std::vector<float4> pt(3 * (size_t)std::max(p0[ts_.N], 1));
+    // an energy configuration's pair carries its virial coefficients
+    // u_a u_b d (xx, yy, zz, xy; yz, zx) in place of (unit vector, d): four
+    // float4 instead of three; the header's first word is the atom's first
+    // float4
+    auto rows_before = [&](int i) {
+      const long nf = std::min(i, ts_.N_force);
+      return 3 * (long)p0[nf] + 4 * ((long)p0[i] - p0[nf]);
+    };
+    const long rows = rows_before(ts_.N);
+    if (rows > INT_MAX) fail("NN2B pair table too large");
+    std::vector<float4> pt((size_t)std::max<long>(rows, 1));
     std::vector<int4> hdr(std::max(ts_.N, 1));
     parallel_for(ts_.N, [&](int i) {
       const int nn = nl_.NN[i], b0 = nl_.start[i];
+      const bool force = i < ts_.N_force;
       std::vector<int> order;
       order.reserve(nn);
       for (int u = 0; u < nn; ++u)
         if (evaluates(i, b0 + u)) order.push_back(b0 + u);
       std::stable_sort(order.begin(), order.end(),
                        [&](int a, int b) { return geo_.bond[a].w < geo_.bond[b].w; });
-      hdr[i] = make_int4(p0[i], (int)order.size(), 0, 0);
-      float4* out = &pt[3 * (size_t)p0[i]];
+      const long first = rows_before(i);
+      hdr[i] = make_int4((int)first, (int)order.size(), 0, 0);
+      float4* out = &pt[(size_t)first];
       for (int b : order) {
         const Vec4 r = geo_.bond[b];
         const double d = r.w, dinv = 1.0 / d;
@@ -462,7 +474,13 @@ void Fitness::setup()
           fc = 0.5 * std::cos(a) + 0.5;
           fcp = -0.5 * (double)NN2B_PI_FACTOR * std::sin(a);
         }
-        *out++ = make_float4((float)(r.x * dinv), (float)(r.y * dinv), (float)(r.z * dinv), r.w);
+        if (force) {
+          *out++ = make_float4((float)(r.x * dinv), (float)(r.y * dinv), (float)(r.z * dinv), r.w);
+        } else {
+          *out++ = make_float4((float)(r.x * r.x * dinv), (float)(r.y * r.y * dinv),
+                               (float)(r.z * r.z * dinv), (float)(r.x * r.y * dinv));
+          *out++ = make_float4((float)(r.y * r.z * dinv), (float)(r.z * r.x * dinv), 0.0f, 0.0f);
+        }
         *out++ = make_float4((float)(2 * t3 - 3 * t2 + 1), (float)(h * (t3 - 2 * t2 + t)),
                              (float)(-2 * t3 + 3 * t2), (float)(h * (t3 - t2)));
         // (the species code t_i + t_j picks the individual's table for it)

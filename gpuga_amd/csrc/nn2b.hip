@@ -126,32 +126,34 @@ struct NN2BPop {
   // The pairs of an atom's table, four (then two, then one) at a time: the
   // pairs' table rows are all loaded before any is evaluated, so the
   // lookups (a gather that depends on the scalar load of the cell) are
-  // waited for once per step rather than once per pair.
-  template <class F>
+  // waited for once per step rather than once per pair.  R float4 per pair,
+  // the last two the Hermite weights and the cell; f(entry, rows, weights,
+  // cell) reads the rest of the entry itself.
+  template <int R, class F>
   __device__ static __forceinline__ void each_pair(const float* __restrict__ tk, long cs,
                                                    const float4* tab, int n, F&& f)
   {
     int k = 0;
     for (; k + 4 <= n; k += 4) {
-      const float4 c0 = ldu(tab, 3 * k + 2), c1 = ldu(tab, 3 * k + 5);
-      const float4 c2 = ldu(tab, 3 * k + 8), c3 = ldu(tab, 3 * k + 11);
+      const float4 c0 = ldu(tab, R * k + R - 1), c1 = ldu(tab, R * (k + 1) + R - 1);
+      const float4 c2 = ldu(tab, R * (k + 2) + R - 1), c3 = ldu(tab, R * (k + 3) + R - 1);
       const Rows r0 = rows(tk, cs, c0), r1 = rows(tk, cs, c1), r2 = rows(tk, cs, c2),
                  r3 = rows(tk, cs, c3);
-      f(ldu(tab, 3 * k), r0, ldu(tab, 3 * k + 1), c0);
-      f(ldu(tab, 3 * k + 3), r1, ldu(tab, 3 * k + 4), c1);
-      f(ldu(tab, 3 * k + 6), r2, ldu(tab, 3 * k + 7), c2);
-      f(ldu(tab, 3 * k + 9), r3, ldu(tab, 3 * k + 10), c3);
+      f(tab + R * k, r0, ldu(tab, R * k + R - 2), c0);
+      f(tab + R * (k + 1), r1, ldu(tab, R * (k + 1) + R - 2), c1);
+      f(tab + R * (k + 2), r2, ldu(tab, R * (k + 2) + R - 2), c2);
+      f(tab + R * (k + 3), r3, ldu(tab, R * (k + 3) + R - 2), c3);
     }
     for (; k + 2 <= n; k += 2) {
-      const float4 u0 = ldu(tab, 3 * k), w0 = ldu(tab, 3 * k + 1), c0 = ldu(tab, 3 * k + 2);
-      const float4 u1 = ldu(tab, 3 * k + 3), w1 = ldu(tab, 3 * k + 4), c1 = ldu(tab, 3 * k + 5);
+      const float4 c0 = ldu(tab, R * k + R - 1), c1 = ldu(tab, R * (k + 1) + R - 1);
+      const float4 w0 = ldu(tab, R * k + R - 2), w1 = ldu(tab, R * (k + 1) + R - 2);
       const Rows r0 = rows(tk, cs, c0), r1 = rows(tk, cs, c1);
-      f(u0, r0, w0, c0);
-      f(u1, r1, w1, c1);
+      f(tab + R * k, r0, w0, c0);
+      f(tab + R * (k + 1), r1, w1, c1);
     }
     if (k < n) {
-      const float4 u = ldu(tab, 3 * k), w = ldu(tab, 3 * k + 1), c = ldu(tab, 3 * k + 2);
-      f(u, rows(tk, cs, c), w, c);
+      const float4 c = ldu(tab, R * k + R - 1), w = ldu(tab, R * k + R - 2);
+      f(tab + R * k, rows(tk, cs, c), w, c);
     }
   }
 
@@ -165,10 +167,12 @@ struct NN2BPop {
     if (it.c < s.Nc_force) {
       for (int i = it.a0; i < it.a1; ++i) {
         const int4 h = ldu(s.pair_hdr, i);
-        const float4* tab = s.pair_tab + 3 * (long)h.x;
+        const float4* tab = s.pair_tab + (long)h.x;
         float fx = 0, fy = 0, fz = 0;
-        each_pair(tk, ex.code_stride, tab, h.y,
-                  [&](const float4& u, const Rows& r, const float4& w, const float4& c) {
+        // a force configuration's pair: (unit vector, d), weights, cell
+        each_pair<3>(tk, ex.code_stride, tab, h.y,
+                     [&](const float4* e, const Rows& r, const float4& w, const float4& c) {
+          const float4 u = ldu(e, 0);
           float p2, G;
           pair(r, w, c, p2, G);
           fx = fmaf(u.x, G, fx);
@@ -183,21 +187,22 @@ struct NN2BPop {
       // each unordered pair once (the table holds j > i), full weight
       for (int i = it.a0; i < it.a1; ++i) {
         const int4 h = ldu(s.pair_hdr, i);
-        const float4* tab = s.pair_tab + 3 * (long)h.x;
+        const float4* tab = s.pair_tab + (long)h.x;
         float e = 0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wyz = 0, wzx = 0;
-        each_pair(tk, ex.code_stride, tab, h.y,
-                  [&](const float4& u, const Rows& r, const float4& w, const float4& c) {
+        // an energy configuration's pair: its virial coefficients u_a u_b d
+        // (xx, yy, zz, xy; yz, zx), weights, cell: one FMA per component
+        each_pair<4>(tk, ex.code_stride, tab, h.y,
+                     [&](const float4* en, const Rows& r, const float4& w, const float4& c) {
+          const float4 v0 = ldu(en, 0), v1 = ldu(en, 1);
           float p2, G;
           pair(r, w, c, p2, G);
-          const float t = u.w * G;
-          const float tx = u.x * t, ty = u.y * t, tz = u.z * t;
           e += p2;
-          wxx = fmaf(-u.x, tx, wxx);
-          wyy = fmaf(-u.y, ty, wyy);
-          wzz = fmaf(-u.z, tz, wzz);
-          wxy = fmaf(-u.x, ty, wxy);
-          wyz = fmaf(-u.y, tz, wyz);
-          wzx = fmaf(-u.z, tx, wzx);
+          wxx = fmaf(-v0.x, G, wxx);
+          wyy = fmaf(-v0.y, G, wyy);
+          wzz = fmaf(-v0.z, G, wzz);
+          wxy = fmaf(-v0.w, G, wxy);
+          wyz = fmaf(-v1.x, G, wyz);
+          wzx = fmaf(-v1.y, G, wzx);
         });
         acc.v[0] += e;
         acc.v[1] += wxx;
