@@ -665,7 +665,9 @@ void Fitness::build_plan(PopPlan& pl, int P, const LaunchShape& sh)
   // has a SIMD of its own (a launch too small to fill the device, so its time
   // is the longest chain rather than the total work), the force
   // configurations' chunks are cut at half the item cost, so that both
-  // chains end together.
+  // chains end together.  (Cut at a third, the waves of a launch of this
+  // kind went from 768 to 984 and every atom took longer: 1.03 against 0.87
+  // us per energy atom.)
   long long force_cost = chunk_cost;
   if (tersoff && ts_.Nc_force > 0) {
     // (a force configuration of several chunks has an assembly item per
@@ -967,10 +969,10 @@ void Fitness::launch_population(PopPlan& pl, hipStream_t st, const float* genes,
       launch_tersoff_loop_population(ctl, ds_.view, loop_, G, pl.block, st);
       break;
     }
-    // the instantiation with the packed path only when the plan has packed items
+    // the instantiation with the packed path only when the plan has packed
+    // items; the one for lone waves when there is at most a workgroup per CU
     launch_tersoff_population(ctl, ds_.view, nn_class(population_neighbors().max_nn), G,
-                              pl.block,
-                              pl.groups_last > 1, st);
+                              pl.block, pl.groups_last > 1, G <= num_cu_, st);
     break;
   }
   }

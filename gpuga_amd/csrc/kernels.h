@@ -26,9 +26,10 @@ void launch_fitness_terms(const DevSet& s, const FitnessNorm& nrm, const float* 
 
 // ---- population kernels (one persistent launch per generation)
 // cls: neighbour-count class of the training set (0: <= 4, 1: <= 8, 2: <= 16);
-// pack: serve the packed-item queue (several atoms per wave)
+// pack: serve the packed-item queue (several atoms per wave); lone: a launch
+// of at most one wave per SIMD (the form that trades registers for latency)
 void launch_tersoff_population(const PopCtl& ctl, const DevSet& s, int cls, int grid, int block,
-                               bool pack, hipStream_t st);
+                               bool pack, bool lone, hipStream_t st);
 // Tersoff sets with more than 16 neighbours within R2: per-bond lane values
 // in LDS (dynamic, tersoff_loop_lds_bytes per workgroup), geometry from the
 // bond and triplet tables of the list in t.

@@ -40,10 +40,11 @@
 // partials in chunk order and writes the configuration's error terms; the last
 // configuration of a group (about sqrt(Nc) of them) sums the group in index
 // order; the last group of a lane block sums the groups and writes the
-// fitness.  Every reduction has a fixed order, so results do not depend on
-// scheduling or on what else runs on the device.  All counters are returned
-// to zero by their last user, so the launch can be replayed from a captured
-// graph without clearing memory.
+// fitness (with one group, its last configuration does).  Every reduction
+// has a fixed order, so results do not depend on scheduling or on what else
+// runs on the device.  All counters are returned to zero by their last user,
+// so the launch can be replayed from a captured graph without clearing
+// memory.
 //
 // Replaces the per-individual host loop of Fitness::compute
 // (src/fitness.cu:316-333) and its reductions (src/fitness.cu:392-555).
@@ -56,7 +57,9 @@ namespace gpuga {
 
 // Configurations per reduction group: about sqrt(Nc), a power of two in
 // [8, 64], so that the group fold and the lane-block fold over the groups
-// are both a few memory round trips.
+// are both a few memory round trips.  (One group for a set of at most 64
+// configurations, with the lane-block fold left out, was slower on a set of
+// 60: the group's last configuration then loads all 60 rows.)
 inline int config_group(int Nc)
 {
   int g = 8;
@@ -205,6 +208,23 @@ __device__ inline void sum_rows(const double* const (&a)[NA], long stride, int n
   for (int x = 0; x < NA; ++x) out[x] = (s[x][0] + s[x][1]) + (s[x][2] + s[x][3]);
 }
 
+// Fitness and terms of the lane block's individuals from the sums of the
+// three error terms over all configurations.
+__device__ inline void write_fitness(const PopCtl& ctl, int k, int lane, const double (&s)[3])
+{
+  const double tE = sqrt(s[0] * ctl.norm.inv_pss);
+  const double tW = sqrt(s[1] * ctl.norm.inv_vss);
+  const double tF = sqrt(s[2] * ctl.norm.inv_fss);
+  const int p = k * WAVE + lane;
+  if (p < ctl.P) {
+    ctl.fitness[p] =
+      (float)(ctl.norm.w_energy * tE + ctl.norm.w_stress * tW + ctl.norm.w_force * tF);
+    ctl.terms[3 * (long)p + 0] = tE;
+    ctl.terms[3 * (long)p + 1] = tW;
+    ctl.terms[3 * (long)p + 2] = tF;
+  }
+}
+
 // Configuration c of lane block k has its error terms (force configurations
 // the force term eF only, the others energy and virial eE, eW): propagate
 // through the group and lane-block reductions, writing the fitness when the
@@ -240,6 +260,10 @@ __device__ inline bool config_done(const PopCtl& ctl, int Nc, int Nc_force, int 
   const double* const ee[2] = {b + (long)fe * WAVE, b + E3 + (long)fe * WAVE};
   sum_rows<2, VALS>(ee, WAVE, g0 + gn - fe, s);
   s[2] = f[0];
+  if (groups == 1) {
+    write_fitness(ctl, k, lane, s);
+    return true;
+  }
   double* gp = ctl.gpart + ((long)k * groups + g) * 3 * WAVE + lane;
   cstore(gp, s[0]);
   cstore(gp + WAVE, s[1]);
@@ -252,17 +276,7 @@ __device__ inline bool config_done(const PopCtl& ctl, int Nc, int Nc_force, int 
   const double* gb = ctl.gpart + (long)k * groups * 3 * WAVE + lane;
   const double* const eg[3] = {gb, gb + WAVE, gb + 2 * WAVE};
   sum_rows<3, VALS>(eg, 3 * WAVE, groups, s);
-  const double tE = sqrt(s[0] * ctl.norm.inv_pss);
-  const double tW = sqrt(s[1] * ctl.norm.inv_vss);
-  const double tF = sqrt(s[2] * ctl.norm.inv_fss);
-  const int p = k * WAVE + lane;
-  if (p < ctl.P) {
-    ctl.fitness[p] =
-      (float)(ctl.norm.w_energy * tE + ctl.norm.w_stress * tW + ctl.norm.w_force * tF);
-    ctl.terms[3 * (long)p + 0] = tE;
-    ctl.terms[3 * (long)p + 1] = tW;
-    ctl.terms[3 * (long)p + 2] = tF;
-  }
+  write_fitness(ctl, k, lane, s);
   return true;
 }
 
@@ -302,17 +316,20 @@ __device__ inline bool fold_chunks(const PopCtl& ctl, unsigned* counter, int c, 
       for (int j = 0; j < B; ++j) acc.v[7] += a[j];
     }
   } else {
-    // three chunks' partials (21 values) per round trip, added in chunk order
-    for (int ch = 0; ch < nch; ch += 3) {
-      double a[7], b[7], d[7];
+    // VALS / 7 chunks' partials per round trip (at least 3), added in chunk
+    // order
+    constexpr int B = VALS / 7 > 3 ? VALS / 7 : 3;
+    for (int ch = 0; ch < nch; ch += B) {
+      double a[B][7];
 #pragma unroll
-      for (int u = 0; u < 7; ++u) {
-        a[u] = cload(base + ch * stride + u * WAVE);
-        b[u] = ch + 1 < nch ? cload(base + (ch + 1) * stride + u * WAVE) : 0.0;
-        d[u] = ch + 2 < nch ? cload(base + (ch + 2) * stride + u * WAVE) : 0.0;
-      }
+      for (int j = 0; j < B; ++j)
 #pragma unroll
-      for (int u = 0; u < 7; ++u) acc.v[u] = ((acc.v[u] + a[u]) + b[u]) + d[u];
+        for (int u = 0; u < 7; ++u)
+          a[j][u] = ch + j < nch ? cload(base + (ch + j) * stride + u * WAVE) : 0.0;
+#pragma unroll
+      for (int j = 0; j < B; ++j)
+#pragma unroll
+        for (int u = 0; u < 7; ++u) acc.v[u] += a[j][u];
     }
   }
   return true;
@@ -399,10 +416,10 @@ __device__ inline double group_sum(double x, int W)
 //   struct Lane; typedef Extra;                     per-lane constants, tables
 //   static Lane make(ctl, p);                       denormalised genes of p
 //   static void chunk(set, q, item, ctl, ex, Acc&)  evaluate atoms [a0, a1)
-//   static constexpr bool kHeaderAhead;             chunk<true> exists: a form
-//                                                   that loads ahead (used by
-//                                                   the instantiation without
-//                                                   packed items)
+//   static constexpr bool kHeaderAhead;             chunk<true, LONE> exists:
+//                                                   a form that loads ahead
+//                                                   (used by the instantiations
+//                                                   without packed items)
 //   static constexpr bool kForcePass;               three-body force assembly
 //   static double force_pass(set, ctl, ex, a0, a1, k, lane)
 //                                                   sum |F-F_ref|^2 over [a0, a1)
@@ -416,8 +433,14 @@ __device__ inline double group_sum(double x, int W)
 // item); the other one never contains the packed code, so its register
 // budget is unaffected.  (Forcing the PACK one to the other's occupancy
 // makes it spill to scratch, and a launch that needs scratch memory costs
-// more than the wave it saves.)
-template <class Pot, bool PACK = false>
+// more than the wave it saves.)  LONE is the instantiation for launches of
+// at most one wave per SIMD, where a wave's time is the latency of its
+// dependent loads and registers are plentiful: with one lane block the
+// individuals' constants load while the fetch is in flight, and a chunk
+// loads each next atom's record during the current one and reads its rows
+// from the wave's LDS copy (Pot::chunk<true, true>; only with kHeaderAhead
+// and without PACK).
+template <class Pot, bool PACK = false, bool LONE = false>
 __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
                                                          typename Pot::Extra ex)
 {
@@ -439,7 +462,15 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
   const int slot_u = wave_first((int)wave_slot);
   if (slot_u < ctl.n_home) q = ldu(ctl.home, slot_u);
   bool skip_fetch = false;
+  const bool early_lanes = LONE && ctl.K == 1;
+  // values loaded per round trip by the reductions (more, in the lone-wave
+  // form, made its launches slower: 48 values, 26.6 against 24.9 us)
+  constexpr int FV = Pot::kFoldValues;
   for (;;) {
+    // With one lane block, every item's individuals are known before the
+    // fetch: their constants load while it is in flight.
+    typename Pot::Lane lanes{};
+    if (early_lanes) lanes = Pot::make(ctl, min(lane, ctl.P - 1));
     // (readfirstlane: the flag is the same in every lane, which the compiler
     // cannot tell, and a per-lane item index would move all the item's
     // values out of scalar registers)
@@ -463,11 +494,16 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
     if (it < 0) {
       // Every queue is empty.  The last wave to finish returns the queue
       // counters to zero (all other waves have stopped fetching by then).
-      const unsigned d = wave_atomic_add(ctl.done, 1u);
-      if (d == total_waves - 1u && lane == 0) {
-        for (int j = 0; j < ctl.nq; ++j) cstore(ctl.fetch + (long)j * kFetchStride, 0u);
-        cstore(ctl.closed, 0ull);
-        cstore(ctl.done, 0u);
+      // With one lane block, the wave that wrote the fitness is not counted:
+      // it fetches nothing more, and the kernel ends with its store.
+      if (!skip_fetch) {
+        const unsigned d = wave_atomic_add(ctl.done, 1u);
+        const unsigned counted = total_waves - (ctl.K == 1 ? 1u : 0u);
+        if (d == counted - 1u && lane == 0) {
+          for (int j = 0; j < ctl.nq; ++j) cstore(ctl.fetch + (long)j * kFetchStride, 0u);
+          cstore(ctl.closed, 0ull);
+          cstore(ctl.done, 0u);
+        }
       }
       break;
     }
@@ -498,13 +534,12 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
         }
       } else {
         // individual of this lane; lanes past P repeat the last one, write nothing
-        const int p = min(k * WAVE + lane, ctl.P - 1);
-        const typename Pot::Lane q = Pot::make(ctl, p);
+        if (!early_lanes) lanes = Pot::make(ctl, min(k * WAVE + lane, ctl.P - 1));
         mark(4);
         if constexpr (Pot::kHeaderAhead && !PACK)
-          Pot::template chunk<true>(set, q, item, ctl, ex, acc);
+          Pot::template chunk<true, LONE>(set, lanes, item, ctl, ex, acc);
         else
-          Pot::chunk(set, q, item, ctl, ex, acc);
+          Pot::chunk(set, lanes, item, ctl, ex, acc);
         mark(5);
       }
       if (Pot::kForcePass && is_force) {
@@ -527,7 +562,7 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
           acc.v[7] = Pot::force_pass(set, ctl, ex, item.a0, item.a1, k, lane);
         }
       } else if (nch > 1) {
-        if (!fold_chunks<Pot::kFoldValues>(ctl, cnt, c, k, item.chunk, nch, lane, is_force, acc))
+        if (!fold_chunks<FV>(ctl, cnt, c, k, item.chunk, nch, lane, is_force, acc))
           continue;
         if (lane == 0) *cnt = 0u;
       }
@@ -549,7 +584,7 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
       mark(5);
       unsigned* cnt2 = &ctl.cnt2[(long)c * ctl.K + k];
       if (nch > 1) {
-        if (!fold_chunks<Pot::kFoldValues>(ctl, cnt2, c, k, item.chunk, nch, lane, true, acc))
+        if (!fold_chunks<FV>(ctl, cnt2, c, k, item.chunk, nch, lane, true, acc))
           continue;
       }
       if (lane == 0) {
@@ -574,7 +609,7 @@ __global__ void __launch_bounds__(256) population_kernel(PopCtl ctl, DevSet set,
     // the launch has finished, so every queue is empty: this wave goes
     // straight to the exit (two memory round trips less at the end of the
     // launch's longest chain).
-    if (config_done<Pot::kFoldValues>(ctl, Nc, set.Nc_force, c, k, lane, eE, eW, eF) &&
+    if (config_done<FV>(ctl, Nc, set.Nc_force, c, k, lane, eE, eW, eF) &&
         ctl.K == 1)
       skip_fetch = true;
   }

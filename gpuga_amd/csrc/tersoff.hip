@@ -10,6 +10,8 @@
 //
 // Formulas: src/minimal_tersoff.cu:62-119 (fR, fA, fc, g), :147-185 (bond
 // order), :250-307 (partial force f_ij), :355-392 (force and virial).
+#include <type_traits>
+
 #include "fastmath.h"
 #include "kernels.h"
 #include "population.h"
@@ -264,7 +266,7 @@ struct TersoffPop {
     // space is issued as a vector load, waited for at every atom)
     const int* H = reinterpret_cast<const int*>(s.rec + (long)i * RS);
     atom_h<FORCE, U>(s, q, i, Hdr{ldx<U>(H, 0), TWO ? ldx<U>(H, 1) : 0, __int_as_float(ldx<U>(H, 3))},
-                     fblk, lane, acc);
+                     s.rec + (long)i * RS, fblk, lane, acc);
   }
 
   // The next atom's record, touched by one vector load while the current
@@ -291,12 +293,20 @@ struct TersoffPop {
                __int_as_float(__builtin_amdgcn_readlane(v, 1))};
   }
 
-  // the atom with its header already loaded
-  template <bool FORCE, bool U>
-  __device__ static void atom_h(const DevSet& s, const TLane& q, int i, const Hdr& h,
-                                float* fblk, int lane, Acc& acc)
+  // Row loads: scalar (U: wave-uniform record in global memory), from the
+  // wave's LDS copy of the record (LR), or per lane.
+  template <bool U, bool LR, class T>
+  __device__ static T ldr(const T* p, long i)
   {
-    const float4* R = s.rec + (long)i * RS;
+    if constexpr (U && !LR) return ldu(p, i);
+    else return p[i];
+  }
+
+  // the atom with its header already loaded; R: its record (in LDS with LR)
+  template <bool FORCE, bool U, bool LR = false>
+  __device__ static void atom_h(const DevSet& s, const TLane& q, int i, const Hdr& h,
+                                const float4* R, float* fblk, int lane, Acc& acc)
+  {
     const int nn = h.nn, unl = h.unl;
     // an atom with the full neighbour bound (every atom of diamond Si) runs
     // a body without neighbour-count guards: no selects, no guarded loads;
@@ -307,13 +317,13 @@ struct TersoffPop {
     if constexpr (U) {
       if (nn == NMAX) {
         if (NMAX <= 4 && h.dmax < q.r1min)
-          atom_body<FORCE, U, true, NMAX <= 4>(s, q, i, R, nn, unl, fblk, lane, acc);
+          atom_body<FORCE, U, true, NMAX <= 4, LR>(s, q, i, R, nn, unl, fblk, lane, acc);
         else
-          atom_body<FORCE, U, true, false>(s, q, i, R, nn, unl, fblk, lane, acc);
+          atom_body<FORCE, U, true, false, LR>(s, q, i, R, nn, unl, fblk, lane, acc);
         return;
       }
     }
-    atom_body<FORCE, U, false, false>(s, q, i, R, nn, unl, fblk, lane, acc);
+    atom_body<FORCE, U, false, false, LR>(s, q, i, R, nn, unl, fblk, lane, acc);
   }
 
   // INNER: every bond inside every lane's R1.  Then fc = 1 and fc' = 0, and
@@ -321,7 +331,7 @@ struct TersoffPop {
   // B_v = A_v fc_v is A_v, the fc' terms of f_ij (and with them dsum) are
   // zero.  The results are the same bits as the general body's, which
   // multiplies by the exact 1 and adds the exact 0.
-  template <bool FORCE, bool U, bool FULL, bool INNER>
+  template <bool FORCE, bool U, bool FULL, bool INNER, bool LR = false>
   __device__ static void atom_body(const DevSet& s, const TLane& q, int i, const float4* R,
                                    int nn, int unl, float* fblk, int lane, Acc& acc)
   {
@@ -337,13 +347,13 @@ struct TersoffPop {
     if constexpr (PRE) {
 #pragma unroll
       for (int u = 0; u < NMAX; ++u) {
-        d[u] = ldx<U>(R, 1 + u).w;
-        dinv[u] = ldx<U>(reinterpret_cast<const float*>(R + INV), u);
+        d[u] = ldr<U, LR>(R, 1 + u).w;
+        dinv[u] = ldr<U, LR>(reinterpret_cast<const float*>(R + INV), u);
       }
     }
-    auto dist = [&](int u) { return PRE ? d[u] : ldx<U>(R, 1 + u).w; };
+    auto dist = [&](int u) { return PRE ? d[u] : ldr<U, LR>(R, 1 + u).w; };
     auto dist_inv = [&](int u) {
-      return PRE ? dinv[u] : ldx<U>(reinterpret_cast<const float*>(R + INV), u);
+      return PRE ? dinv[u] : ldr<U, LR>(reinterpret_cast<const float*>(R + INV), u);
     };
     float fc[NMAX], fcp[NMAX], fcb[NMAX], A[NMAX], B[NMAX], pf[NMAX];
 #pragma unroll
@@ -370,7 +380,7 @@ struct TersoffPop {
 #pragma unroll
         for (int v = u + 1; v < NMAX; ++v)
           if (FULL || v < nn) {
-            xs[u][v] = ldx<U>(R, 1 + NMAX + u * NMAX + v).x - q.h;
+            xs[u][v] = ldr<U, LR>(R, 1 + NMAX + u * NMAX + v).x - q.h;
             x2[u][v] = xs[u][v] * xs[u][v];
           }
     }
@@ -384,7 +394,7 @@ struct TersoffPop {
         if constexpr (PRE && !SYM) {
 #pragma unroll
           for (int v = 0; v < NMAX; ++v)
-            if (v != u) cs[v] = ldx<U>(R, 1 + NMAX + u * NMAX + v).x;
+            if (v != u) cs[v] = ldr<U, LR>(R, 1 + NMAX + u * NMAX + v).x;
         }
         float zeta = 0.0f;
 #pragma unroll
@@ -393,7 +403,7 @@ struct TersoffPop {
             if constexpr (SYM) {
               zeta = fmaf(fcb[v], x2sym(u, v), zeta);
             } else {
-              const float x = (PRE ? cs[v] : ldx<U>(R, 1 + NMAX + u * NMAX + v).x) - q.h;
+              const float x = (PRE ? cs[v] : ldr<U, LR>(R, 1 + NMAX + u * NMAX + v).x) - q.h;
               zeta = fmaf(fcb[v], x * x, zeta);
             }
           }
@@ -433,13 +443,13 @@ struct TersoffPop {
         if constexpr (PRE) {
 #pragma unroll
           for (int v = 0; v < NMAX; ++v)
-            if (v != u) row[v] = ldx<U>(R, 1 + NMAX + u * NMAX + v);
+            if (v != u) row[v] = ldr<U, LR>(R, 1 + NMAX + u * NMAX + v);
         }
         float cx = 0.0f, cy = 0.0f, cz = 0.0f, dsum = 0.0f;
 #pragma unroll
         for (int v = 0; v < NMAX; ++v) {
           if ((FULL || v < nn) && v != u) {
-            const float4 t = PRE ? row[v] : ldx<U>(R, 1 + NMAX + u * NMAX + v);
+            const float4 t = PRE ? row[v] : ldr<U, LR>(R, 1 + NMAX + u * NMAX + v);
             const float x = SYM ? xsym(u, v) : t.x - q.h;
             const float w = (INNER ? A[u] + A[v] : fmaf(A[u], fc[v], B[v])) * x;
             cx = fmaf(w, t.y, cx);
@@ -451,7 +461,7 @@ struct TersoffPop {
             }
           }
         }
-        const float4 r = ldx<U>(R, 1 + u);
+        const float4 r = ldr<U, LR>(R, 1 + u);
         const float coef = INNER ? 0.5f * pf[u] : 0.5f * (pf[u] - fcp[u] * (q.beta * dsum));
         const float hc = INNER ? q.beta : fc[u] * q.beta;
         const float fx = coef * r.x - hc * cx;
@@ -460,7 +470,7 @@ struct TersoffPop {
         if (FORCE) {
           // f_ij goes to the slot of the reverse bond, where atom j reads it
           // back as f_ji; F_i's own part sum_j f_ij stays in registers
-          const int rb = ldx<U>(reinterpret_cast<const int*>(R + REV), u);
+          const int rb = ldr<U, LR>(reinterpret_cast<const int*>(R + REV), u);
           float* o = fblk + (size_t)rb * 3 * WAVE + lane;
           cstore(o, fx);
           cstore(o + WAVE, fy);
@@ -497,17 +507,68 @@ struct TersoffPop {
     }
   }
 
+  // Lone-wave form: a record spread over the wave, word l in lane l and
+  // word 64 + l in lane l (two vector loads).
+  static constexpr int RW = RS * 4; // words per record
+  static_assert(NMAX > 4 || (RW >= WAVE && RW <= 2 * WAVE), "record layout of the lone-wave form");
+  struct Spread {
+    int lo, hi;
+  };
+  __device__ static Spread spread_load(const DevSet& s, int i)
+  {
+    const int* w = reinterpret_cast<const int*>(s.rec + (long)i * RS);
+    const int l = __lane_id();
+    return Spread{w[l], w[min(WAVE + l, RW - 1)]};
+  }
+  __device__ static void spread_store(float4* buf, const Spread& v)
+  {
+    int* w = reinterpret_cast<int*>(buf);
+    const int l = __lane_id();
+    w[l] = v.lo;
+    if (WAVE + l < RW) w[WAVE + l] = v.hi;
+  }
+  __device__ static Hdr spread_header(const Spread& v)
+  {
+    return Hdr{__builtin_amdgcn_readlane(v.lo, 0), TWO ? __builtin_amdgcn_readlane(v.lo, 1) : 0,
+               __int_as_float(__builtin_amdgcn_readlane(v.lo, 3))};
+  }
+
   // HDR: load the next atom's header during the current atom (four-bond
   // records; the kernel instantiation without the packed path, whose
-  // register budget has room for it)
+  // register budget has room for it).  LONE (launches of at most one wave
+  // per SIMD, where a wave's time is the latency of its loads): the next
+  // atom's whole record is loaded during the current atom, spread over the
+  // wave, and written to the wave's LDS buffer at its end; the rows are read
+  // from there (an LDS round trip per atom instead of a memory one).
   static constexpr bool kHeaderAhead = true;
-  template <bool HDR = false>
+  template <bool HDR = false, bool LONE = false>
   __device__ static void chunk(const DevSet& s, const TLane& q, const WorkItem& it,
                                const PopCtl& ctl, int, Acc& acc)
   {
     const int lane = __lane_id();
     const unsigned r = dispatch_round();
-    if constexpr (NMAX > 4 || !HDR) {
+    if constexpr (LONE && NMAX <= 4) {
+      // two records per wave of a 256-thread workgroup (the kernel's bound)
+      __shared__ float4 buf[256 / WAVE][2][RS];
+      const int w = wave_first((int)(threadIdx.x / WAVE));
+      Spread cur = spread_load(s, it.a0);
+      spread_store(buf[w][0], cur);
+      auto run = [&](auto force) {
+        constexpr bool F = decltype(force)::value;
+        float* fblk = F ? ctl.bondf + (size_t)it.k * ctl.B_force * 3 * WAVE : nullptr;
+        for (int i = it.a0; i < it.a1; ++i) {
+          rotate_priority(r + (unsigned)i);
+          const int b = (i - it.a0) & 1;
+          const Hdr h = spread_header(cur);
+          const Spread nxt = spread_load(s, min(i + 1, it.a1 - 1));
+          atom_h<F, true, true>(s, q, i, h, buf[w][b], fblk, lane, acc);
+          spread_store(buf[w][b ^ 1], nxt);
+          cur = nxt;
+        }
+      };
+      if (it.c < s.Nc_force) run(std::true_type{});
+      else run(std::false_type{});
+    } else if constexpr (NMAX > 4 || !HDR) {
       if (it.c < s.Nc_force) {
         float* fblk = ctl.bondf + (size_t)it.k * ctl.B_force * 3 * WAVE;
         for (int i = it.a0; i < it.a1; ++i) {
@@ -530,14 +591,14 @@ struct TersoffPop {
           rotate_priority(r + (unsigned)i);
           const Hdr h = header_of(hv);
           hv = ahead_load(s, min(i + 1, it.a1 - 1));
-          atom_h<true, true>(s, q, i, h, fblk, lane, acc);
+          atom_h<true, true>(s, q, i, h, s.rec + (long)i * RS, fblk, lane, acc);
         }
       } else {
         for (int i = it.a0; i < it.a1; ++i) {
           rotate_priority(r + (unsigned)i);
           const Hdr h = header_of(hv);
           hv = ahead_load(s, min(i + 1, it.a1 - 1));
-          atom_h<false, true>(s, q, i, h, nullptr, lane, acc);
+          atom_h<false, true>(s, q, i, h, s.rec + (long)i * RS, nullptr, lane, acc);
         }
       }
     }
@@ -812,25 +873,35 @@ int tersoff_loop_occupancy(bool two, int max_nn, int block)
 
 template <int NMAX, bool TWO>
 static void launch_t(const PopCtl& ctl, const DevSet& s, int grid, int block, bool pack,
-                     hipStream_t st)
+                     bool lone, hipStream_t st)
 {
-  if (pack) population_kernel<TersoffPop<NMAX, TWO>, true><<<grid, block, 0, st>>>(ctl, s, 0);
-  else population_kernel<TersoffPop<NMAX, TWO>, false><<<grid, block, 0, st>>>(ctl, s, 0);
+  if (pack) {
+    population_kernel<TersoffPop<NMAX, TWO>, true><<<grid, block, 0, st>>>(ctl, s, 0);
+    return;
+  }
+  if constexpr (NMAX == 4) {
+    if (lone) {
+      population_kernel<TersoffPop<NMAX, TWO>, false, true><<<grid, block, 0, st>>>(ctl, s, 0);
+      return;
+    }
+  }
+  population_kernel<TersoffPop<NMAX, TWO>, false><<<grid, block, 0, st>>>(ctl, s, 0);
 }
 
 // cls: 0 for at most 4 neighbours per atom, 1 for 8, 2 for 16; pack: the
-// packed-item instantiation
+// packed-item instantiation; lone: the instantiation for launches of one
+// wave per SIMD (four-bond records, no packed items)
 void launch_tersoff_population(const PopCtl& ctl, const DevSet& s, int cls, int grid, int block,
-                               bool pack, hipStream_t st)
+                               bool pack, bool lone, hipStream_t st)
 {
   const bool two = ctl.V == 13;
   switch (cls * 2 + (two ? 1 : 0)) {
-  case 0: launch_t<4, false>(ctl, s, grid, block, pack, st); break;
-  case 1: launch_t<4, true>(ctl, s, grid, block, pack, st); break;
-  case 2: launch_t<8, false>(ctl, s, grid, block, pack, st); break;
-  case 3: launch_t<8, true>(ctl, s, grid, block, pack, st); break;
-  case 4: launch_t<16, false>(ctl, s, grid, block, pack, st); break;
-  default: launch_t<16, true>(ctl, s, grid, block, pack, st); break;
+  case 0: launch_t<4, false>(ctl, s, grid, block, pack, lone, st); break;
+  case 1: launch_t<4, true>(ctl, s, grid, block, pack, lone, st); break;
+  case 2: launch_t<8, false>(ctl, s, grid, block, pack, lone, st); break;
+  case 3: launch_t<8, true>(ctl, s, grid, block, pack, lone, st); break;
+  case 4: launch_t<16, false>(ctl, s, grid, block, pack, lone, st); break;
+  default: launch_t<16, true>(ctl, s, grid, block, pack, lone, st); break;
   }
   GPUGA_HIP(hipGetLastError());
 }
