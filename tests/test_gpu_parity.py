@@ -201,6 +201,23 @@ def test_launch_shape_does_not_change_results(G, golden, name):
     np.testing.assert_allclose(ta, tref, rtol=1e-5)
 
 
+@pytest.mark.parametrize("name,P", [("si_diamond", 128), ("gaas", 128), ("si_diamond", 50)])
+def test_lone_wave_form_same_bits(G, golden, name, P):
+    """A grid of at most one workgroup per CU runs the four-bond Tersoff
+    records with the lone-wave instantiation (each next atom's record staged
+    through LDS; with one lane block, P = 50, the individuals' constants
+    loaded during the first fetch), four workgroups per CU with the default
+    one: the same arithmetic on the same values, so the same bits."""
+    print_case("lone-wave form", name, P)
+    genes = genes_for(9 if name == "si_diamond" else 13, P, 97)
+    out = {}
+    for bpc in (1, 4):
+        with G.Fitness(golden(name), chunk_cost=200, blocks_per_cu=bpc) as f:
+            out[bpc] = f.compute(genes, terms=True)
+    assert np.array_equal(out[1][1], out[4][1])
+    assert np.array_equal(out[1][0], out[4][0])
+
+
 @pytest.mark.parametrize("name", ["si_diamond", "gaas", "hfo2"])
 def test_cross_wave_handoff_stress(G, golden, name):
     """Tiny work items split every configuration into many chunks, so chunk
