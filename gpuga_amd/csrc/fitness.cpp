@@ -137,7 +137,8 @@ size_t DeviceSet::bytes() const
          rev.bytes() + ptype.bytes() + tri0.bytes() + bond.bytes() + tri.bytes() +
          ri_tab.bytes() + rec.bytes() + pair_hdr.bytes() + pair_tab.bytes() + pe_ref.bytes() +
          virial_ref.bytes() + force_ref.bytes() + p_nn.bytes() + p_b0.bytes() + p_nl.bytes() +
-         p_rev.bytes() + p_ptype.bytes() + p_tri0.bytes() + p_bond.bytes() + p_tri.bytes();
+         p_rev.bytes() + p_ptype.bytes() + p_tri0.bytes() + p_bond.bytes() + p_tri.bytes() +
+         ri_cconst.bytes() + ri_fconst.bytes();
 }
 
 void PopPlan::drop_ga_graphs()
@@ -383,15 +384,20 @@ void Fitness::setup()
     ds_.ri_tab.upload(tab, st);
     // population form: per atom the pairs it evaluates (all bonds for force
     // configurations, those with j > i for the others), grouped by pair type
-    // t_i + t_j.  A force configuration's pair: unit vector and d, then the
-    // factors plus 6 d^-7 (two float4); an energy configuration's pair: d and
-    // the virial coefficients u_a u_b d (xx, yy, zz, then xy, yz, zx), then
-    // the factors (three float4: the virial then takes one FMA per
-    // component).  The header's first word is the atom's first float4.
+    // t_i + t_j.  A force configuration's pair: unit vector and d (one
+    // float4); an energy configuration's pair: d and the virial coefficients
+    // u_a u_b d (xx, yy, zz, then xy, yz, zx; two float4: the virial then
+    // takes one FMA per component).  The header's first word is the atom's
+    // first float4.  Of a pair's terms (src/ri.cu:77-110) only the
+    // Buckingham exponential and the Morse term are non-linear in the
+    // parameters; the others are a parameter times a function of d, so their
+    // sums over a configuration (energy, virial) or over an atom's bonds
+    // (force) are taken here once, in FP64, and the kernel adds the
+    // parameters' combination of them once per configuration or atom.
     const std::vector<int> p0 = pair_offsets();
     auto rows_before = [&](int i) { // float4 rows of the atoms before i
       const long nf = std::min(i, ts_.N_force);
-      return 2 * (long)p0[nf] + 3 * ((long)p0[i] - p0[nf]);
+      return (long)p0[nf] + 2 * ((long)p0[i] - p0[nf]);
     };
     const long rows = rows_before(ts_.N);
     if (rows > INT_MAX) fail("rigid-ion pair table too large");
@@ -408,8 +414,7 @@ void Fitness::setup()
           const int b = b0 + u;
           if (ptype[b] != t || !evaluates(i, b)) continue;
           const Vec4 r = geo_.bond[b];
-          const double d = r.w, dinv = 1.0 / d;
-          const double d6 = std::pow(dinv, 6);
+          const double dinv = 1.0 / r.w;
           if (force) {
             *out++ = make_float4((float)(r.x * dinv), (float)(r.y * dinv), (float)(r.z * dinv), r.w);
           } else {
@@ -419,13 +424,65 @@ void Fitness::setup()
             *out++ = make_float4((float)(r.x * r.y * dinv), (float)(r.y * r.z * dinv),
                                  (float)(r.z * r.x * dinv), 0.0f);
           }
-          *out++ = make_float4((float)d6, (float)(6.0 * d6 * dinv), tab[b].y, tab[b].z);
           ++cnt[t];
         }
       hdr[i] = make_int4((int)first, cnt[0], cnt[1], cnt[2]);
     });
     ds_.pair_tab.upload(pt, st);
     ds_.pair_hdr.upload(hdr, st);
+    // f_6 = 6 d^-7, f_c = erfc(a d)/d, h = -(d/dr) f_c (a = 0.2, as above)
+    auto factors = [](double d, double& d6, double& f6, double& fc, double& h) {
+      const double dinv = 1.0 / d;
+      d6 = std::pow(dinv, 6);
+      f6 = 6.0 * d6 * dinv;
+      fc = std::erfc(0.2 * d) * dinv;
+      h = fc * dinv + 0.225675833419103 * std::exp(-0.04 * d * d) * dinv;
+    };
+    std::vector<double> cc((size_t)std::max(ts_.Nc, 1) * 3 * kRiTypeConst, 0.0);
+    parallel_for(ts_.Nc, [&](int c) {
+      if (c < ts_.Nc_force) return;
+      double* out = &cc[(size_t)c * 3 * kRiTypeConst];
+      for (int i = ts_.Na_sum[c]; i < ts_.Na_sum[c] + ts_.Na[c]; ++i)
+        for (int b = nl_.start[i]; b < nl_.start[i] + nl_.NN[i]; ++b) {
+          if (!evaluates(i, b)) continue;
+          const Vec4 r = geo_.bond[b];
+          const double d = r.w;
+          double d6, f6, fc, h;
+          factors(d, d6, f6, fc, h);
+          const double v[6] = {r.x * r.x / d, r.y * r.y / d, r.z * r.z / d,
+                               r.x * r.y / d, r.y * r.z / d, r.z * r.x / d};
+          double* o = out + (size_t)ptype[b] * kRiTypeConst;
+          o[0] += 1.0;
+          o[1] += d;
+          o[2] += fc;
+          o[3] += d6;
+          for (int a = 0; a < 6; ++a) {
+            o[4 + a] += v[a];
+            o[10 + a] += v[a] * f6;
+            o[16 + a] += v[a] * h;
+          }
+        }
+    });
+    ds_.ri_cconst.upload(cc, st);
+    std::vector<float> fcon((size_t)std::max(ts_.N_force, 1) * kRiAtomConst, 0.0f);
+    parallel_for(ts_.N_force, [&](int i) {
+      double o[kRiAtomConst] = {};
+      for (int b = nl_.start[i]; b < nl_.start[i] + nl_.NN[i]; ++b) {
+        const Vec4 r = geo_.bond[b];
+        const double d = r.w;
+        double d6, f6, fc, h;
+        factors(d, d6, f6, fc, h);
+        const double u[3] = {r.x / d, r.y / d, r.z / d};
+        for (int x = 0; x < 3; ++x) {
+          double* w = o + ptype[b] * 9 + x * 3;
+          w[0] += u[x] * f6;
+          w[1] += u[x];
+          w[2] += u[x] * h;
+        }
+      }
+      for (int k = 0; k < kRiAtomConst; ++k) fcon[(size_t)i * kRiAtomConst + k] = (float)o[k];
+    });
+    ds_.ri_fconst.upload(fcon, st);
   }
   if (spec_.nn2b()) {
     // population form: per atom the pairs it evaluates (all bonds for force
@@ -518,6 +575,8 @@ void Fitness::setup()
   v.rec = ds_.rec.get();
   v.pair_hdr = ds_.pair_hdr.get();
   v.pair_tab = ds_.pair_tab.get();
+  v.ri_cconst = ds_.ri_cconst.get();
+  v.ri_fconst = ds_.ri_fconst.get();
   v.pe_ref = ds_.pe_ref.get();
   v.virial_ref = ds_.virial_ref.get();
   v.force_ref = ds_.force_ref.get();

@@ -53,6 +53,8 @@ struct DeviceSet {
   DeviceBuffer<float4> p_bond, p_tri;
   DeviceBuffer<int4> pair_hdr;
   DeviceBuffer<float> pe_ref, virial_ref, force_ref;
+  DeviceBuffer<double> ri_cconst;
+  DeviceBuffer<float> ri_fconst;
   DevSet view{};
   size_t bytes() const;
 };

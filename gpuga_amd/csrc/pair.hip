@@ -111,30 +111,6 @@ __device__ inline RILane ri_lane(const RIParams& p)
   return q;
 }
 
-// One pair of type T (src/ri.cu:77-110) from its table entry
-// u = (unit vector, d), f = (d^-6, 6 d^-7, erfc(a d)/d, -(d/dr)(erfc(a d)/d)):
-// g = U'(d) and, when ENERGY, p = U(d).
-template <int T, bool ENERGY>
-__device__ __forceinline__ void ri_pair_pop(const RILane& q, float d, const float4& f, float& p,
-                                            float& g)
-{
-  const float aex = q.a[T] * fexp2(-d * q.b2[T]);
-  g = fmaf(q.c[T], f.y, -fmaf(aex, q.b[T], q.k1[T]));
-  g = fmaf(-q.qq[T], f.w, g);
-  if (ENERGY) {
-    p = fmaf(q.qq[T], f.z, aex);
-    p = fmaf(-q.c[T], f.x, p);
-    p = fmaf(-q.k1[T], d, p) - q.k0[T];
-  }
-  if (T == 1 && q.morse) {
-    const float x = d - q.r0;
-    const float fr = q.mR * fexp2(q.kR2 * x);
-    const float fa = q.mA * fexp2(q.kA2 * x);
-    if (ENERGY) p += fr - fa;
-    g += q.kR * fr - q.kA * fa;
-  }
-}
-
 struct RIPop {
   static constexpr bool kForcePass = false;
   static constexpr int kFoldValues = 12;
@@ -151,16 +127,19 @@ struct RIPop {
     return ri_lane(ri_params(x));
   }
 
-  // The pairs of type T of one atom: F_i += u g (force configurations) or
-  // E += U, W -= u (x) u d g over the unordered pairs (the others).  The
-  // geometry is wave-uniform (scalar loads); four pairs per step (their
-  // loads issue together and are waited for once), then two, then one.
-  // (float4 rows per pair: two for a force configuration's pair, three for
-  // an energy configuration's)
+  // The pairs of type T of one atom: the parts of F_i += u g (force
+  // configurations) or of E += U, W -= u (x) u d g over the unordered pairs
+  // (the others) that are not linear in the parameters: the Buckingham
+  // exponential and, for unlike pairs, the Morse term (the linear parts come
+  // from the set's sums, linear_force and linear_config).  The geometry is
+  // wave-uniform (scalar loads); four pairs per step (their loads issue
+  // together and are waited for once), then two, then one.  (float4 rows
+  // per pair: one for a force configuration's pair, two for an energy
+  // configuration's)
   template <int T, bool FORCE>
   __device__ static void pairs(const RILane& q, const float4* tab, int n, float* a)
   {
-    constexpr int R = FORCE ? 2 : 3;
+    constexpr int R = FORCE ? 1 : 2;
     int k = 0;
     for (; k + 4 <= n; k += 4) {
       float4 r[4][R];
@@ -188,21 +167,31 @@ struct RIPop {
     }
   }
 
+  // The non-linear terms of one pair (src/ri.cu:77-110): U = A e^(-d/rho)
+  // (+ Morse) and g = U' = -A e^(-d/rho)/rho (+ Morse'); F += u g, or E += U,
+  // W -= v g
   template <int T, bool FORCE>
-  __device__ static __forceinline__ void one(const RILane& q, const float4 (&r)[FORCE ? 2 : 3],
+  __device__ static __forceinline__ void one(const RILane& q, const float4 (&r)[FORCE ? 1 : 2],
                                              float* a)
   {
-    float p, g;
+    const float d = FORCE ? r[0].w : r[0].x;
+    const float aex = q.a[T] * fexp2(-d * q.b2[T]);
+    float p = aex, g = -aex * q.b[T];
+    if (T == 1 && q.morse) {
+      const float x = d - q.r0;
+      const float fr = q.mR * fexp2(q.kR2 * x);
+      const float fa = q.mA * fexp2(q.kA2 * x);
+      p += fr - fa;
+      g += q.kR * fr - q.kA * fa;
+    }
     if constexpr (FORCE) {
-      // r[0] = (unit vector, d), r[1] = factors
-      ri_pair_pop<T, false>(q, r[0].w, r[1], p, g);
+      // r[0] = (unit vector, d)
       a[0] = fmaf(r[0].x, g, a[0]);
       a[1] = fmaf(r[0].y, g, a[1]);
       a[2] = fmaf(r[0].z, g, a[2]);
     } else {
       // r[0] = (d, u_x u_x d, u_y u_y d, u_z u_z d), r[1] = (u_x u_y d,
-      // u_y u_z d, u_z u_x d, 0), r[2] = factors
-      ri_pair_pop<T, true>(q, r[0].x, r[2], p, g);
+      // u_y u_z d, u_z u_x d, 0)
       a[0] += p;
       a[1] = fmaf(-r[0].y, g, a[1]);
       a[2] = fmaf(-r[0].z, g, a[2]);
@@ -210,6 +199,39 @@ struct RIPop {
       a[4] = fmaf(-r[1].x, g, a[4]);
       a[5] = fmaf(-r[1].y, g, a[5]);
       a[6] = fmaf(-r[1].z, g, a[6]);
+    }
+  }
+
+  // The linear part of F_i: per pair type t and component x,
+  // C_t sum u_x 6 d^-7 - k1_t sum u_x - qq_t sum u_x h (dispersion, and the
+  // shifted Coulomb g = -qq (h + dv_rc) with k1 = qq dv_rc).
+  __device__ static void linear_force(const RILane& q, const float* w, float* a)
+  {
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        const float* s = w + t * 9 + x * 3;
+        a[x] = fmaf(q.c[t], ldu(s, 0), a[x]);
+        a[x] = fmaf(-q.k1[t], ldu(s, 1), a[x]);
+        a[x] = fmaf(-q.qq[t], ldu(s, 2), a[x]);
+      }
+  }
+
+  // The linear part of a configuration's energy and virial (FP64, once per
+  // configuration and lane block): per pair type t,
+  //   E += qq_t sum erfc(a d)/d - C_t sum d^-6 - k1_t sum d - k0_t n
+  //   W += -C_t sum v 6 d^-7 + k1_t sum v + qq_t sum v h
+  __device__ static void linear_config(const RILane& q, const double* w, Acc& acc)
+  {
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const double* s = w + t * kRiTypeConst;
+      const double qq = q.qq[t], c = q.c[t], k1 = q.k1[t], k0 = q.k0[t];
+      acc.v[0] += ((qq * ldu(s, 2) - c * ldu(s, 3)) - k1 * ldu(s, 1)) - k0 * ldu(s, 0);
+#pragma unroll
+      for (int m = 0; m < 6; ++m)
+        acc.v[1 + m] += (k1 * ldu(s, 4 + m) - c * ldu(s, 10 + m)) + qq * ldu(s, 16 + m);
     }
   }
 
@@ -226,21 +248,24 @@ struct RIPop {
         const int4 h = ldu(s.pair_hdr, i);
         const float4* tab = s.pair_tab + (long)h.x;
         float a[3] = {0.0f, 0.0f, 0.0f};
+        linear_force(q, s.ri_fconst + (long)i * kRiAtomConst, a);
         pairs<0, true>(q, tab, h.y, a);
-        pairs<1, true>(q, tab + 2 * (long)h.y, h.z, a);
-        pairs<2, true>(q, tab + 2 * (long)(h.y + h.z), h.w, a);
+        pairs<1, true>(q, tab + (long)h.y, h.z, a);
+        pairs<2, true>(q, tab + (long)(h.y + h.z), h.w, a);
         const float dx = a[0] - ldu(s.force_ref, i), dy = a[1] - ldu(s.force_ref, i + N),
                     dz = a[2] - ldu(s.force_ref, i + 2 * N);
         acc.v[7] += (double)dx * dx + (double)dy * dy + (double)dz * dz;
       }
     } else {
+      // the configuration's linear part goes with its first chunk
+      if (it.chunk == 0) linear_config(q, s.ri_cconst + (long)it.c * 3 * kRiTypeConst, acc);
       for (int i = it.a0; i < it.a1; ++i) {
         const int4 h = ldu(s.pair_hdr, i);
         const float4* tab = s.pair_tab + (long)h.x;
         float a[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         pairs<0, false>(q, tab, h.y, a);
-        pairs<1, false>(q, tab + 3 * (long)h.y, h.z, a);
-        pairs<2, false>(q, tab + 3 * (long)(h.y + h.z), h.w, a);
+        pairs<1, false>(q, tab + 2 * (long)h.y, h.z, a);
+        pairs<2, false>(q, tab + 2 * (long)(h.y + h.z), h.w, a);
 #pragma unroll
         for (int u = 0; u < 7; ++u) acc.v[u] += (double)a[u];
       }
