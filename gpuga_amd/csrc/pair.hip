@@ -85,6 +85,10 @@ struct RILane {
   float a[3], b[3], b2[3], c[3], qq[3], k0[3], k1[3];
   float mR, kR, kR2, mA, kA, kA2, r0;
   int morse; // wave-uniform: some lane of the wave has a Morse term
+  // wave-uniform: bit t set when some lane's pairs of type t have a term
+  // evaluated per pair (A_t != 0, or the Morse term on unlike pairs); the
+  // others add exact zeros and are left out
+  int live;
 };
 
 __device__ inline RILane ri_lane(const RIParams& p)
@@ -108,6 +112,8 @@ __device__ inline RILane ri_lane(const RIParams& p)
   q.kA2 = p.kA * LOG2E;
   q.r0 = p.r0;
   q.morse = __any(p.mR != 0.0f || p.mA != 0.0f);
+  q.live = (__any(p.a[0] != 0.0f) ? 1 : 0) | (__any(p.a[1] != 0.0f) || q.morse ? 2 : 0) |
+           (__any(p.a[2] != 0.0f) ? 4 : 0);
   return q;
 }
 
@@ -249,9 +255,9 @@ struct RIPop {
         const float4* tab = s.pair_tab + (long)h.x;
         float a[3] = {0.0f, 0.0f, 0.0f};
         linear_force(q, s.ri_fconst + (long)i * kRiAtomConst, a);
-        pairs<0, true>(q, tab, h.y, a);
-        pairs<1, true>(q, tab + (long)h.y, h.z, a);
-        pairs<2, true>(q, tab + (long)(h.y + h.z), h.w, a);
+        if (q.live & 1) pairs<0, true>(q, tab, h.y, a);
+        if (q.live & 2) pairs<1, true>(q, tab + (long)h.y, h.z, a);
+        if (q.live & 4) pairs<2, true>(q, tab + (long)(h.y + h.z), h.w, a);
         const float dx = a[0] - ldu(s.force_ref, i), dy = a[1] - ldu(s.force_ref, i + N),
                     dz = a[2] - ldu(s.force_ref, i + 2 * N);
         acc.v[7] += (double)dx * dx + (double)dy * dy + (double)dz * dz;
@@ -263,9 +269,9 @@ struct RIPop {
         const int4 h = ldu(s.pair_hdr, i);
         const float4* tab = s.pair_tab + (long)h.x;
         float a[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        pairs<0, false>(q, tab, h.y, a);
-        pairs<1, false>(q, tab + 2 * (long)h.y, h.z, a);
-        pairs<2, false>(q, tab + 2 * (long)(h.y + h.z), h.w, a);
+        if (q.live & 1) pairs<0, false>(q, tab, h.y, a);
+        if (q.live & 2) pairs<1, false>(q, tab + 2 * (long)h.y, h.z, a);
+        if (q.live & 4) pairs<2, false>(q, tab + 2 * (long)(h.y + h.z), h.w, a);
 #pragma unroll
         for (int u = 0; u < 7; ++u) acc.v[u] += (double)a[u];
       }
