@@ -5,7 +5,8 @@
 //    SGPRs, so branches on it are scalar; used only for buffers that no
 //    kernel writes while it runs (the training-set tables).
 //  * exp2 / log2 / reciprocal: the CDNA transcendental instructions
-//    (v_exp_f32, v_log_f32, v_rcp_f32, about 1 ulp on normal inputs).
+//    (v_exp_f32, v_log_f32, v_rcp_f32, about 1 ulp on normal inputs), and
+//    tanh from them (the NN2B tabulation's hidden layers).
 //  * cos(pi t), sin(pi t) for t in [0, 1] (the cutoff taper): reduced to
 //    y = pi (t - 1/2) in [-pi/2, pi/2] and evaluated from the Taylor series of
 //    sin y (to y^11) and cos y (to y^12), whose truncation errors on that
@@ -41,6 +42,14 @@ constexpr float PI_F = 3.14159265358979f;
 __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float flog2(float x) { return __builtin_amdgcn_logf(x); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// tanh x = 1 - 2 / (e^(2x) + 1) from exp2 and the reciprocal: absolute
+// error about 2e-7 (the subtraction from 1 sets it near x = 0); -1 and +1
+// for large |x| (e^(2x) flushes to 0 or overflows to +inf)
+__device__ __forceinline__ float ftanh(float x)
+{
+  return 1.0f - 2.0f * frcp(fexp2(x * (2.0f * LOG2E)) + 1.0f);
+}
 
 // sin(pi t) and cos(pi t) for 0 <= t <= 1
 __device__ __forceinline__ void sincospi01(float t, float& s, float& c)

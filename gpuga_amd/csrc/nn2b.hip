@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(256) nn2b_tabulate_f32(PopCtl ctl, NN2BGrid gr
     if (m < n) {
       // the species-pair input as a shift of the bias (none for one code)
       const float bm = grid.codes > 1 ? b0[m] - ws[m] * (float)code : b0[m];
-      const float t = tanhf(w0[m] * d - bm);
+      const float t = ftanh(w0[m] * d - bm);
       const float s = 1.0f - t * t;
       x1[m] = t;
       y1[m] = s * w0[m];                      // d x1 / dr
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) nn2b_tabulate_f32(PopCtl ctl, NN2BGrid gr
         zdd += wm * y2[m];
       }
     }
-    const float x2 = tanhf(z - b1[a]);
+    const float x2 = ftanh(z - b1[a]);
     const float s = 1.0f - x2 * x2;
     pv += w2[a] * x2;
     pd += w2[a] * s * zd;

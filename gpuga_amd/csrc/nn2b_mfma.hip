@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(64) nn2b_tabulate_bf16(PopCtl ctl, NN2BGrid gr
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const float w = w0m[8 * s + j];
-        const float t = tanhf(w * d - b0m[8 * s + j]);
+        const float t = ftanh(w * d - b0m[8 * s + j]);
         const float sg = 1.0f - t * t;
         split_bf16(t, AXh, AXl, j);
         split_bf16(sg * w, AYh, AYl, j);
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(64) nn2b_tabulate_bf16(PopCtl ctl, NN2BGrid gr
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int row = 8 * (r >> 2) + 4 * kg + (r & 3);
-      const float x2 = tanhf(Z[r] - b1c);
+      const float x2 = ftanh(Z[r] - b1c);
       const float s = 1.0f - x2 * x2;
       red[0][row][col] = w2c * x2;
       red[1][row][col] = w2c * s * Zd[r];
